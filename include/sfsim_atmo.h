@@ -1,0 +1,150 @@
+/*
+ * libsfsim_atmo: MI355X (gfx950) atmospheric lookup-table precompute for sfsim.
+ *
+ * C ABI only: plain structs of doubles, int shapes, float buffers and sizes.  Every entry point
+ * returns 0 on success or a negative SFSIM_ATMO_ERR_* code; the message of the last failure on a
+ * context is returned by sfsim_atmo_last_error.  No pointer passed in is kept after a call returns.
+ * Calls on one context are serialised on that context's HIP stream and are synchronous for the
+ * caller; different contexts may be used from different threads.
+ *
+ * The reference builds these tables in Clojure with closures over sfsim.atmosphere functions
+ * (src/clj/sfsim/atmosphere_lut.clj:43-105); closures cannot cross a C ABI, so the boundary is
+ * the table level: each `interpolate-function` call of atmosphere_lut.clj:74-101 is one entry
+ * point here, and sfsim_atmo_generate_luts replaces generate-atmosphere-luts as a whole.
+ *
+ * Table layout of every float buffer: grid order of the reference's nested vectors (row-major
+ * over the space's shape, dimension 0 outermost), 3 floats (RGB) per entry, i.e. exactly
+ * (pack-matrices table) of matrix.clj:130-134.  4D tables handed to generate_luts' outputs are
+ * additionally tiled 4D->2D as image.clj:299-312 does for the .scatter files.
+ */
+#ifndef SFSIM_ATMO_H
+#define SFSIM_ATMO_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFSIM_ATMO_OK 0
+#define SFSIM_ATMO_ERR_ARGUMENT (-1)  /* invalid size, step count, null pointer, unknown space */
+#define SFSIM_ATMO_ERR_DEVICE (-2)    /* no usable gfx950 device / HIP runtime error */
+#define SFSIM_ATMO_ERR_ALLOC (-3)     /* device or host allocation failed */
+#define SFSIM_ATMO_ERR_NONFINITE (-4) /* a parameter or input table holds NaN/Inf */
+#define SFSIM_ATMO_ERR_IO (-5)        /* file could not be written/read completely */
+
+/* Planet map of atmosphere.clj:64-67 + sphere.clj:25 (:sfsim.sphere/centre, /radius,
+ * :sfsim.atmosphere/height, /brightness).  The centre must be the origin: the reference's
+ * is-above-horizon? (atmosphere.clj:95-102) and every table space assume it. */
+typedef struct sfsim_atmo_planet {
+  double centre[3];
+  double radius;
+  double height;
+  double brightness[3];
+} sfsim_atmo_planet;
+
+/* Scattering component of atmosphere.clj:35-39.  quotient: 1.0 when the reference omits it;
+ * g: 0.0 when omitted. */
+typedef struct sfsim_atmo_scatter {
+  double scatter_base[3];
+  double scatter_scale;
+  double scatter_quotient;
+  double scatter_g;
+} sfsim_atmo_scatter;
+
+/* Everything generate-atmosphere-luts hard-codes (atmosphere_lut.clj:20-62). */
+typedef struct sfsim_atmo_params {
+  sfsim_atmo_planet planet;
+  sfsim_atmo_scatter mie;      /* scatter = [mie rayleigh] (atmosphere_lut.clj:63) */
+  sfsim_atmo_scatter rayleigh;
+  double intensity[3];
+  int transmittance_shape[2];    /* [height elevation] */
+  int surface_radiance_shape[2]; /* [height sun-elevation] */
+  int scatter_shape[4];          /* [height elevation sun-elevation heading] */
+  int ray_steps;                 /* integral-ray steps; also the half-sphere steps of surface-radiance */
+  int sphere_steps;              /* integral-sphere steps of point-scatter */
+  int iterations;                /* scattering-order iterations */
+} sfsim_atmo_params;
+
+typedef struct sfsim_atmo_ctx sfsim_atmo_ctx;
+
+/* Lifecycle (the reference's native library keeps global state behind jolt_init/jolt_destroy,
+ * src/c/sfsim/jolt.cc:137-182; here the state lives in a context). */
+int sfsim_atmo_create(int device, sfsim_atmo_ctx** ctx);
+int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx);
+const char* sfsim_atmo_last_error(const sfsim_atmo_ctx* ctx);
+
+/* T = (interpolate-function transmittance transmittance-space) table (atmosphere_lut.clj:66-67,74;
+ * atmosphere.clj:114-128, 291-319).  out: transmittance_shape entries x 3 floats. */
+int sfsim_atmo_transmittance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* out);
+
+/* dE0 = surface-radiance-base table (atmosphere_lut.clj:68-69,75; atmosphere.clj:131-137, 335-365). */
+int sfsim_atmo_surface_radiance_base_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* out);
+
+/* First-order tables of atmosphere_lut.clj:70-73,77-78: ray-scatter of point-scatter-component for
+ * Rayleigh (with phase) and of strength-component for Mie (no phase) (atmosphere.clj:162-200),
+ * computed in one pass.  Either output may be NULL. */
+int sfsim_atmo_single_scatter_tables(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* rayleigh_out,
+                                     float* mie_strength_out);
+
+/* dJ = point-scatter table (atmosphere_lut.clj:88,91; atmosphere.clj:203-222).
+ * dS is the previous order's ray-scatter table.  When mie_strength is non-NULL the first-iteration
+ * function of atmosphere_lut.clj:79-84 is used instead: dS(q) + mie_strength(q) * phase(mie, w.l).
+ * dE is the previous order's surface-radiance table. */
+int sfsim_atmo_point_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const float* dS,
+                                   const float* mie_strength, const float* dE, float* out);
+
+/* dS' = ray-scatter table of an interpolated dJ table (atmosphere_lut.clj:92,94; atmosphere.clj:192-200). */
+int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const float* dJ, float* out);
+
+/* dE' = surface-radiance table of dS (atmosphere_lut.clj:89,93; atmosphere.clj:225-230), half-sphere
+ * steps = ray_steps.  mie_strength as for point_scatter. */
+int sfsim_atmo_surface_radiance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const float* dS,
+                                      const float* mie_strength, float* out);
+
+/* Re-sampling f(backward(i)) of interpolated tables on one space, the accumulations of
+ * atmosphere_lut.clj:95-97 (b != NULL: a + b) and the final make-lookup-table calls of 98-101
+ * (b == NULL).  space: 2 = transmittance, 3 = surface radiance, 4 = ray/point scatter. */
+#define SFSIM_ATMO_SPACE_TRANSMITTANCE 2
+#define SFSIM_ATMO_SPACE_SURFACE_RADIANCE 3
+#define SFSIM_ATMO_SPACE_SCATTER 4
+int sfsim_atmo_resample_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int space, const float* a,
+                              const float* b, float* out);
+
+/* generate-atmosphere-luts (atmosphere_lut.clj:43-105) end to end.  Outputs in the exact byte
+ * order of the four .scatter files: transmittance [h][e] RGB, surface-radiance [h][s] RGB,
+ * ray-scatter and mie-strength 4D->2D tiled (rows h*S+s, cols e*A+a) RGB.  Any output may be NULL. */
+int sfsim_atmo_generate_luts(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* transmittance,
+                             float* surface_radiance, float* ray_scatter, float* mie_strength);
+
+/* Device-resident build for callers that keep the tables on the GPU (and for timing): runs the
+ * whole generate-atmosphere-luts pipeline into context-owned device buffers without host copies;
+ * sfsim_atmo_fetch then copies one final table out (which: 0 = transmittance, 1 = surface
+ * radiance, 2 = ray scatter, 3 = mie strength; layouts as generate_luts). */
+int sfsim_atmo_build(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p);
+int sfsim_atmo_fetch(sfsim_atmo_ctx* ctx, int which, float* out, size_t count);
+
+/* Kernel durations of the last sfsim_atmo_build, measured with HIP events on the context stream
+ * and summed per kernel kind: ms[k] milliseconds and launches[k] launches of kind k, for
+ * k < min(n, number of kinds); returns the number of kinds.  Kind names: sfsim_atmo_stage_name(k). */
+int sfsim_atmo_stage_timings(const sfsim_atmo_ctx* ctx, double* ms, int* launches, int n);
+const char* sfsim_atmo_stage_name(int kind);
+
+/* Work the kernels perform for one build of p (extinction samples, 4D and 2D lookups), counted on
+ * the device by an instrumented build (not the timed one).  counts: [0] extinction samples of the
+ * single-scatter kernel, [1] extinction samples of all kernels, [2] 4D lookups, [3] 2D lookups. */
+int sfsim_atmo_count_work(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, unsigned long long* counts);
+
+/* util.clj:227-240 spit-floats / util.clj:188-203 slurp-floats: raw little-endian float32, no header. */
+int sfsim_atmo_write_scatter(const char* path, const float* data, size_t count);
+int sfsim_atmo_read_scatter(const char* path, float* data, size_t count);
+
+/* Number of directions of integral-sphere (half = 0) / integral-half-sphere (half = 1) with the
+ * given steps (sphere.clj:70-105). */
+int sfsim_atmo_sphere_directions(int half, int steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFSIM_ATMO_H */

@@ -1,0 +1,308 @@
+"""Host-side mirror of sfsim's LUT precompute interface over libsfsim_atmo (C ABI, ctypes).
+
+Names and argument meaning follow the reference: ``generate_atmosphere_luts`` is
+src/clj/sfsim/atmosphere_lut.clj:43-105, the table functions are the ``interpolate-function``
+calls of atmosphere_lut.clj:74-101, parameter dictionaries use the keys of atmosphere.clj:35-39,
+64-67 (planet: ``centre``, ``radius``, ``height``, ``brightness``; component: ``scatter-base``,
+``scatter-scale``, ``scatter-quotient``, ``scatter-g``).
+
+Every call runs on the GPU through the HIP library; there is no CPU fallback.  A missing library
+or device raises ``AtmoError``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsfsim_atmo.so")
+
+OK, ERR_ARGUMENT, ERR_DEVICE, ERR_ALLOC, ERR_NONFINITE, ERR_IO = 0, -1, -2, -3, -4, -5
+SPACE_TRANSMITTANCE, SPACE_SURFACE_RADIANCE, SPACE_SCATTER = 2, 3, 4
+
+# Every symbol include/sfsim_atmo.h declares.
+EXPORTS = (
+    "sfsim_atmo_create", "sfsim_atmo_destroy", "sfsim_atmo_last_error", "sfsim_atmo_transmittance_table",
+    "sfsim_atmo_surface_radiance_base_table", "sfsim_atmo_single_scatter_tables", "sfsim_atmo_point_scatter_table",
+    "sfsim_atmo_ray_scatter_table", "sfsim_atmo_surface_radiance_table", "sfsim_atmo_resample_table",
+    "sfsim_atmo_generate_luts", "sfsim_atmo_build", "sfsim_atmo_fetch", "sfsim_atmo_stage_timings",
+    "sfsim_atmo_stage_name", "sfsim_atmo_count_work", "sfsim_atmo_write_scatter", "sfsim_atmo_read_scatter",
+    "sfsim_atmo_sphere_directions",
+)
+
+
+class AtmoError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"libsfsim_atmo error {code}: {message}")
+        self.code = code
+
+
+class Planet(ctypes.Structure):
+    _fields_ = [("centre", ctypes.c_double * 3), ("radius", ctypes.c_double), ("height", ctypes.c_double),
+                ("brightness", ctypes.c_double * 3)]
+
+
+class Scatter(ctypes.Structure):
+    _fields_ = [("scatter_base", ctypes.c_double * 3), ("scatter_scale", ctypes.c_double),
+                ("scatter_quotient", ctypes.c_double), ("scatter_g", ctypes.c_double)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("planet", Planet), ("mie", Scatter), ("rayleigh", Scatter), ("intensity", ctypes.c_double * 3),
+                ("transmittance_shape", ctypes.c_int * 2), ("surface_radiance_shape", ctypes.c_int * 2),
+                ("scatter_shape", ctypes.c_int * 4), ("ray_steps", ctypes.c_int), ("sphere_steps", ctypes.c_int),
+                ("iterations", ctypes.c_int)]
+
+
+# atmosphere_lut.clj:20-40
+EARTH = {"centre": (0.0, 0.0, 0.0), "radius": 6378000.0, "height": 35000.0, "brightness": (0.3, 0.3, 0.3)}
+MIE = {"scatter-base": (2e-5, 2e-5, 2e-5), "scatter-scale": 1200.0, "scatter-g": 0.76, "scatter-quotient": 0.9}
+RAYLEIGH = {"scatter-base": (5.8e-6, 13.5e-6, 33.1e-6), "scatter-scale": 8000.0}
+
+
+def make_params(planet=None, mie=None, rayleigh=None, transmittance_shape=(64, 255), surface_radiance_shape=(16, 63),
+                scatter_shape=(32, 127, 32, 8), ray_steps=100, sphere_steps=15, iterations=5,
+                intensity=(1.0, 1.0, 1.0)):
+    """Parameters of one LUT build; defaults are the reference's production values
+    (atmosphere_lut.clj:20-62)."""
+    planet = dict(EARTH, **(planet or {}))
+    mie = dict(MIE, **(mie or {})) if mie is not None else dict(MIE)
+    rayleigh = dict(RAYLEIGH, **(rayleigh or {})) if rayleigh is not None else dict(RAYLEIGH)
+    P = Params()
+    P.planet.centre[:] = list(planet["centre"])
+    P.planet.radius = planet["radius"]
+    P.planet.height = planet["height"]
+    P.planet.brightness[:] = list(planet["brightness"])
+    for dst, src in ((P.mie, mie), (P.rayleigh, rayleigh)):
+        dst.scatter_base[:] = list(src["scatter-base"])
+        dst.scatter_scale = src["scatter-scale"]
+        dst.scatter_quotient = src.get("scatter-quotient", 1.0)
+        dst.scatter_g = src.get("scatter-g", 0.0)
+    P.intensity[:] = list(intensity)
+    P.transmittance_shape[:] = list(transmittance_shape)
+    P.surface_radiance_shape[:] = list(surface_radiance_shape)
+    P.scatter_shape[:] = list(scatter_shape)
+    P.ray_steps = ray_steps
+    P.sphere_steps = sphere_steps
+    P.iterations = iterations
+    return P
+
+
+def workload(name):
+    """Named workloads.  ``production``: the reference's own parameters (atmosphere_lut.clj:20-62).
+    ``earth100-64``: the 100 km / 64-step transmittance grid; ``full-5-order``: the 5-order set at
+    T 64x256, E 16x64, S 32x64x32x8, 64 ray steps, 14 sphere steps (66 directions), g = 0.76."""
+    earth100 = {"height": 100000.0}
+    if name == "production":
+        return make_params()
+    if name == "earth100-64":
+        return make_params(planet=earth100, transmittance_shape=(64, 64), surface_radiance_shape=(16, 64),
+                           scatter_shape=(8, 16, 8, 4), ray_steps=64, sphere_steps=14, iterations=0)
+    if name == "full-5-order":
+        return make_params(planet=earth100, transmittance_shape=(64, 256), surface_radiance_shape=(16, 64),
+                           scatter_shape=(32, 64, 32, 8), ray_steps=64, sphere_steps=14, iterations=5)
+    if name == "large-10-order":
+        return make_params(planet=earth100, transmittance_shape=(128, 512), surface_radiance_shape=(32, 128),
+                           scatter_shape=(64, 128, 64, 16), ray_steps=128, sphere_steps=28, iterations=10)
+    raise KeyError(name)
+
+
+def entries(P):
+    """LUT entries of a full set: T + E + S + M."""
+    s = P.scatter_shape
+    return (P.transmittance_shape[0] * P.transmittance_shape[1] + P.surface_radiance_shape[0] *
+            P.surface_radiance_shape[1] + 2 * s[0] * s[1] * s[2] * s[3])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AtmoError(ERR_DEVICE, f"{LIB_PATH} is not built (python -m sfsim_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, fp, pp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(Params)
+        L.sfsim_atmo_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.sfsim_atmo_destroy.argtypes = [vp]
+        L.sfsim_atmo_last_error.argtypes = [vp]
+        L.sfsim_atmo_last_error.restype = ctypes.c_char_p
+        L.sfsim_atmo_transmittance_table.argtypes = [vp, pp, fp]
+        L.sfsim_atmo_surface_radiance_base_table.argtypes = [vp, pp, fp]
+        L.sfsim_atmo_single_scatter_tables.argtypes = [vp, pp, fp, fp]
+        L.sfsim_atmo_point_scatter_table.argtypes = [vp, pp, fp, fp, fp, fp]
+        L.sfsim_atmo_ray_scatter_table.argtypes = [vp, pp, fp, fp]
+        L.sfsim_atmo_surface_radiance_table.argtypes = [vp, pp, fp, fp, fp]
+        L.sfsim_atmo_resample_table.argtypes = [vp, pp, ctypes.c_int, fp, fp, fp]
+        L.sfsim_atmo_generate_luts.argtypes = [vp, pp, fp, fp, fp, fp]
+        L.sfsim_atmo_build.argtypes = [vp, pp]
+        L.sfsim_atmo_fetch.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
+        L.sfsim_atmo_stage_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                               ctypes.c_int]
+        L.sfsim_atmo_stage_name.argtypes = [ctypes.c_int]
+        L.sfsim_atmo_stage_name.restype = ctypes.c_char_p
+        L.sfsim_atmo_count_work.argtypes = [vp, pp, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.sfsim_atmo_write_scatter.argtypes = [ctypes.c_char_p, fp, ctypes.c_size_t]
+        L.sfsim_atmo_read_scatter.argtypes = [ctypes.c_char_p, fp, ctypes.c_size_t]
+        L.sfsim_atmo_sphere_directions.argtypes = [ctypes.c_int, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a, entries_):
+    if a is None:
+        return None
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32)).reshape(-1)
+    if arr.size != entries_ * 3:
+        raise AtmoError(ERR_ARGUMENT, f"table has {arr.size} floats, expected {entries_ * 3}")
+    return arr
+
+
+def sphere_directions(half, steps):
+    return lib().sfsim_atmo_sphere_directions(int(half), steps)
+
+
+def write_scatter(path, data):
+    arr = np.ascontiguousarray(np.asarray(data, dtype=np.float32)).reshape(-1)
+    rc = lib().sfsim_atmo_write_scatter(os.fsencode(path), _fp(arr), arr.size)
+    if rc:
+        raise AtmoError(rc, f"could not write {path}")
+
+
+def read_scatter(path, count):
+    out = np.zeros(count, dtype=np.float32)
+    rc = lib().sfsim_atmo_read_scatter(os.fsencode(path), _fp(out), count)
+    if rc:
+        raise AtmoError(rc, f"could not read {count} floats from {path}")
+    return out
+
+
+class Atmosphere:
+    """One library context on one GPU (``sfsim_atmo_create``)."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        rc = lib().sfsim_atmo_create(device, ctypes.byref(self._ctx))
+        if rc:
+            raise AtmoError(rc, f"no usable gfx950 device {device}")
+
+    def close(self):
+        if self._ctx:
+            lib().sfsim_atmo_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise AtmoError(rc, lib().sfsim_atmo_last_error(self._ctx).decode())
+
+    # ---------------------------------------------------------------- tables
+    def transmittance_table(self, P):
+        out = np.zeros(tuple(P.transmittance_shape) + (3,), np.float32)
+        self._check(lib().sfsim_atmo_transmittance_table(self._ctx, ctypes.byref(P), _fp(out)))
+        return out
+
+    def surface_radiance_base_table(self, P):
+        out = np.zeros(tuple(P.surface_radiance_shape) + (3,), np.float32)
+        self._check(lib().sfsim_atmo_surface_radiance_base_table(self._ctx, ctypes.byref(P), _fp(out)))
+        return out
+
+    def single_scatter_tables(self, P):
+        """(Rayleigh single scatter R1, Mie strength M1)."""
+        shape = tuple(P.scatter_shape) + (3,)
+        r1, m1 = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+        self._check(lib().sfsim_atmo_single_scatter_tables(self._ctx, ctypes.byref(P), _fp(r1), _fp(m1)))
+        return r1, m1
+
+    def point_scatter_table(self, P, dS, dE, mie_strength=None):
+        n = int(np.prod(P.scatter_shape))
+        ne = P.surface_radiance_shape[0] * P.surface_radiance_shape[1]
+        a, m, e = _f32(dS, n), _f32(mie_strength, n), _f32(dE, ne)
+        out = np.zeros(tuple(P.scatter_shape) + (3,), np.float32)
+        self._check(lib().sfsim_atmo_point_scatter_table(self._ctx, ctypes.byref(P), _fp(a), _fp(m), _fp(e), _fp(out)))
+        return out
+
+    def ray_scatter_table(self, P, dJ):
+        n = int(np.prod(P.scatter_shape))
+        j = _f32(dJ, n)
+        out = np.zeros(tuple(P.scatter_shape) + (3,), np.float32)
+        self._check(lib().sfsim_atmo_ray_scatter_table(self._ctx, ctypes.byref(P), _fp(j), _fp(out)))
+        return out
+
+    def surface_radiance_table(self, P, dS, mie_strength=None):
+        n = int(np.prod(P.scatter_shape))
+        a, m = _f32(dS, n), _f32(mie_strength, n)
+        out = np.zeros(tuple(P.surface_radiance_shape) + (3,), np.float32)
+        self._check(lib().sfsim_atmo_surface_radiance_table(self._ctx, ctypes.byref(P), _fp(a), _fp(m), _fp(out)))
+        return out
+
+    def resample_table(self, P, space, a, b=None):
+        shape = {2: P.transmittance_shape, 3: P.surface_radiance_shape, 4: P.scatter_shape}.get(space)
+        if shape is None:
+            raise AtmoError(ERR_ARGUMENT, f"unknown space {space}")
+        n = int(np.prod(shape))
+        aa, bb = _f32(a, n), _f32(b, n)
+        out = np.zeros(tuple(shape) + (3,), np.float32)
+        self._check(lib().sfsim_atmo_resample_table(self._ctx, ctypes.byref(P), space, _fp(aa), _fp(bb), _fp(out)))
+        return out
+
+    # ---------------------------------------------------------------- whole build
+    def generate_atmosphere_luts(self, P):
+        """atmosphere_lut.clj:43-105: the four tables in .scatter file layout, as float32 arrays
+        shaped like the 2D images the files hold."""
+        T = np.zeros((P.transmittance_shape[0], P.transmittance_shape[1], 3), np.float32)
+        E = np.zeros((P.surface_radiance_shape[0], P.surface_radiance_shape[1], 3), np.float32)
+        s = P.scatter_shape
+        S = np.zeros((s[0] * s[2], s[1] * s[3], 3), np.float32)
+        M = np.zeros_like(S)
+        self._check(lib().sfsim_atmo_generate_luts(self._ctx, ctypes.byref(P), _fp(T), _fp(E), _fp(S), _fp(M)))
+        return {"transmittance": T, "surface-radiance": E, "ray-scatter": S, "mie-strength": M}
+
+    def build(self, P):
+        self._check(lib().sfsim_atmo_build(self._ctx, ctypes.byref(P)))
+
+    def fetch(self, P, which):
+        s = P.scatter_shape
+        shapes = [(P.transmittance_shape[0], P.transmittance_shape[1], 3),
+                  (P.surface_radiance_shape[0], P.surface_radiance_shape[1], 3),
+                  (s[0] * s[2], s[1] * s[3], 3), (s[0] * s[2], s[1] * s[3], 3)]
+        out = np.zeros(shapes[which], np.float32)
+        self._check(lib().sfsim_atmo_fetch(self._ctx, which, _fp(out), out.size))
+        return out
+
+    def stage_timings(self):
+        """{kernel kind: (milliseconds, launches)} of the last build()."""
+        ms = (ctypes.c_double * 16)()
+        n = (ctypes.c_int * 16)()
+        kinds = lib().sfsim_atmo_stage_timings(self._ctx, ms, n, 16)
+        return {lib().sfsim_atmo_stage_name(k).decode(): (ms[k], n[k]) for k in range(kinds)}
+
+    def count_work(self, P):
+        c = (ctypes.c_ulonglong * 4)()
+        self._check(lib().sfsim_atmo_count_work(self._ctx, ctypes.byref(P), c))
+        return {"single_scatter_samples": c[0], "extinction_samples": c[1], "lookups4": c[2], "lookups2": c[3]}
+
+
+def write_scatter_files(directory, luts):
+    """atmosphere_lut.clj:102-105: data/atmosphere/<name>.scatter"""
+    os.makedirs(directory, exist_ok=True)
+    for name, table in luts.items():
+        write_scatter(os.path.join(directory, f"{name}.scatter"), table)

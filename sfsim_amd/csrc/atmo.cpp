@@ -1,0 +1,662 @@
+// C ABI of libsfsim_atmo: contexts, parameter validation, quadrature direction sets, the
+// generate-atmosphere-luts driver (atmosphere_lut.clj:43-105) and the per-table entry points.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sfsim_atmo.h"
+#include "device_params.hpp"
+#include "geometry.hpp"
+#include "kernels.hpp"
+
+using namespace sfsim;
+
+namespace {
+
+enum Kind {
+  kTransmittance = 0,
+  kSurfaceRadianceBase,
+  kSingleScatter,
+  kPointScatter,
+  kSurfaceRadiance,
+  kRayScatter,
+  kAccumulate,
+  kFinal,
+  kNumKinds
+};
+
+const char* const kKindNames[kNumKinds] = {"transmittance", "surface_radiance_base", "single_scatter",
+                                           "point_scatter", "surface_radiance",      "ray_scatter",
+                                           "accumulate",    "final_resample"};
+
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct sfsim_atmo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  // tables (float4 per entry)
+  DeviceBuffer T, dE[2], Eacc[2], R1, M1, dJ, dS[2], Sacc[2];
+  // packed outputs (RGB floats): 0 transmittance, 1 surface radiance, 2 ray scatter (tiled), 3 mie (tiled)
+  DeviceBuffer out[4];
+  size_t out_count[4] = {0, 0, 0, 0};
+  bool built = false;
+  // stage-entry staging
+  DeviceBuffer stage_in[3], stage_out4, stage_rgb;
+  // quadrature directions
+  DeviceBuffer sphere_dirs, half_dirs;
+  int sphere_key = -1, half_key = -1, sphere_n = 0, half_n = 0;
+  DeviceBuffer counters;
+  // timing
+  std::vector<hipEvent_t> events;
+  std::vector<int> event_kind;
+  double kind_ms[kNumKinds] = {};
+  int kind_launches[kNumKinds] = {};
+};
+
+namespace {
+
+int fail(sfsim_atmo_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  return code;
+}
+
+int hip_fail(sfsim_atmo_ctx* ctx, const char* what, hipError_t e) {
+  int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SFSIM_ATMO_ERR_ALLOC : SFSIM_ATMO_ERR_DEVICE;
+  return fail(ctx, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                 \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return hip_fail(ctx, #expr, e_); \
+  } while (0)
+
+#define TRY(expr)             \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+int ensure(sfsim_atmo_ctx* ctx, DeviceBuffer& b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return 0;
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  hipError_t e = hipMalloc(&b.ptr, bytes < 256 ? 256 : bytes);
+  if (e != hipSuccess) return hip_fail(ctx, "hipMalloc", e);
+  b.bytes = bytes < 256 ? 256 : bytes;
+  return 0;
+}
+
+void release(DeviceBuffer& b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+bool finite3(const double* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+long long count2(const int* s) { return (long long)s[0] * s[1]; }
+long long count4(const int* s) { return (long long)s[0] * s[1] * s[2] * s[3]; }
+
+int validate(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  if (!p) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "params is NULL");
+  const sfsim_atmo_planet& pl = p->planet;
+  if (!finite3(pl.centre) || !std::isfinite(pl.radius) || !std::isfinite(pl.height) || !finite3(pl.brightness) ||
+      !finite3(p->intensity))
+    return fail(ctx, SFSIM_ATMO_ERR_NONFINITE, "planet or intensity holds a non-finite value");
+  const sfsim_atmo_scatter* comps[2] = {&p->mie, &p->rayleigh};
+  for (const sfsim_atmo_scatter* c : comps)
+    if (!finite3(c->scatter_base) || !std::isfinite(c->scatter_scale) || !std::isfinite(c->scatter_quotient) ||
+        !std::isfinite(c->scatter_g))
+      return fail(ctx, SFSIM_ATMO_ERR_NONFINITE, "scattering component holds a non-finite value");
+  if (pl.centre[0] != 0.0 || pl.centre[1] != 0.0 || pl.centre[2] != 0.0)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "planet centre must be the origin (atmosphere.clj:95-102 assumes it)");
+  if (!(pl.radius > 0.0) || !(pl.height > 0.0)) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "radius and height must be > 0");
+  for (const sfsim_atmo_scatter* c : comps) {
+    if (!(c->scatter_scale > 0.0)) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "scatter-scale must be > 0");
+    if (!(c->scatter_quotient > 0.0)) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "scatter-quotient must be > 0");
+    if (!(std::fabs(c->scatter_g) < 1.0)) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "|scatter-g| must be < 1");
+  }
+  const int* shapes[3] = {p->transmittance_shape, p->surface_radiance_shape, p->scatter_shape};
+  const int dims[3] = {2, 2, 4};
+  for (int s = 0; s < 3; ++s)
+    for (int d = 0; d < dims[s]; ++d)
+      if (shapes[s][d] < 2 || shapes[s][d] > 65536)
+        return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "every table dimension must be in [2, 65536]");
+  if (count4(p->scatter_shape) > (1LL << 31) / 4 || count2(p->transmittance_shape) > (1LL << 30) ||
+      count2(p->surface_radiance_shape) > (1LL << 30))
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "table too large");
+  if (p->ray_steps < 1 || p->ray_steps > kMaxSteps)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "ray_steps must be in [1, 256]");
+  if (p->iterations < 0 || p->iterations > 1000) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "iterations must be in [0, 1000]");
+  return 0;
+}
+
+// Direction sets need rings: integral-sphere has steps >> 1, integral-half-sphere steps >> 2
+// (sphere.clj:96-105); with none the reference's reduction over no samples fails.
+int validate_sphere(sfsim_atmo_ctx* ctx, int steps) {
+  if ((steps >> 1) < 1 || sfsim_atmo_sphere_directions(0, steps) > kMaxSphereDirs)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sphere_steps must give 1..2048 directions (steps >= 2)");
+  return 0;
+}
+int validate_half_sphere(sfsim_atmo_ctx* ctx, int steps) {
+  if ((steps >> 2) < 1) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "ray_steps must be >= 4 for the surface-radiance half sphere");
+  return 0;
+}
+
+DevParams make_dev_params(const sfsim_atmo_params* p) {
+  DevParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.g.R = p->planet.radius;
+  P.g.H = p->planet.height;
+  P.g.Rt = p->planet.radius + p->planet.height;
+  P.g.rho_max = std::sqrt(sqr(P.g.Rt) - sqr(P.g.R));
+  const double log2e = 1.4426950408889634;
+  for (int c = 0; c < 3; ++c) {
+    P.ext_mie[c] = (float)(p->mie.scatter_base[c] / p->mie.scatter_quotient);
+    P.ext_ray[c] = (float)(p->rayleigh.scatter_base[c] / p->rayleigh.scatter_quotient);
+    P.sca_mie[c] = (float)p->mie.scatter_base[c];
+    P.sca_ray[c] = (float)p->rayleigh.scatter_base[c];
+    P.brightness_over_pi[c] = (float)(p->planet.brightness[c] / M_PI);
+    P.intensity[c] = (float)p->intensity[c];
+  }
+  P.k_mie = (float)(-log2e / p->mie.scatter_scale);
+  P.k_ray = (float)(-log2e / p->rayleigh.scatter_scale);
+  P.inv_scale_mie = 1.0 / p->mie.scatter_scale;
+  P.inv_scale_ray = 1.0 / p->rayleigh.scatter_scale;
+  P.g_mie = (float)p->mie.scatter_g;
+  P.g_ray = (float)p->rayleigh.scatter_g;
+  for (int d = 0; d < 2; ++d) {
+    P.nT[d] = p->transmittance_shape[d];
+    P.nE[d] = p->surface_radiance_shape[d];
+  }
+  for (int d = 0; d < 4; ++d) P.nS[d] = p->scatter_shape[d];
+  P.ray_steps = p->ray_steps;
+  P.graze_height = 2000.0;
+  return P;
+}
+
+// sphere.clj:62-105 for the normal (1 0 0) of every grid point: rings theta_i = range (i + 1/2) / N,
+// weight (cos(theta - d) - cos(theta + d)) 2 pi / K_i, K_i = ceil(sin(theta_i) steps), phi_j = 2 pi (j + 1/2) / K_i,
+// direction x n + y o1 + z o2 with the frame of oriented-matrix (matrix.clj:207-213, quaternion.clj:166-171).
+std::vector<Dir> sphere_directions(int half, int steps) {
+  std::vector<Dir> dirs;
+  int theta_steps = half ? (steps >> 2) : (steps >> 1);
+  double theta_range = half ? M_PI / 2 : M_PI;
+  double n[3] = {1, 0, 0};
+  // orthogonal: basis vector with the smallest |n . e| (first on ties), o1 = normalize(n x e), o2 = n x o1
+  int best = 0;
+  for (int i = 1; i < 3; ++i)
+    if (std::fabs(n[i]) < std::fabs(n[best])) best = i;
+  double e[3] = {0, 0, 0};
+  e[best] = 1;
+  double o1[3] = {n[1] * e[2] - n[2] * e[1], n[2] * e[0] - n[0] * e[2], n[0] * e[1] - n[1] * e[0]};
+  double len = std::sqrt(o1[0] * o1[0] + o1[1] * o1[1] + o1[2] * o1[2]);
+  for (double& v : o1) v /= len;
+  double o2[3] = {n[1] * o1[2] - n[2] * o1[1], n[2] * o1[0] - n[0] * o1[2], n[0] * o1[1] - n[1] * o1[0]};
+  double delta2 = theta_range / theta_steps / 2;
+  for (int i = 0; i < theta_steps; ++i) {
+    double theta = theta_range * ((0.5 + i) / theta_steps);
+    double factor = std::cos(theta - delta2) - std::cos(theta + delta2);
+    int ringsteps = (int)std::ceil(std::sin(theta) * steps);
+    double weight = (2 * M_PI) / ringsteps;
+    double ct = std::cos(theta), st = std::sin(theta);
+    for (int j = 0; j < ringsteps; ++j) {
+      double phi = 2 * M_PI * ((0.5 + j) / ringsteps);
+      double x = ct, y = st * std::cos(phi), z = st * std::sin(phi);
+      Dir d;
+      d.x = n[0] * x + o1[0] * y + o2[0] * z;
+      d.y = n[1] * x + o1[1] * y + o2[1] * z;
+      d.z = n[2] * x + o1[2] * y + o2[2] * z;
+      d.w = weight * factor;
+      dirs.push_back(d);
+    }
+  }
+  return dirs;
+}
+
+int ensure_dirs(sfsim_atmo_ctx* ctx, int half, int steps) {
+  int& key = half ? ctx->half_key : ctx->sphere_key;
+  int& count = half ? ctx->half_n : ctx->sphere_n;
+  DeviceBuffer& buf = half ? ctx->half_dirs : ctx->sphere_dirs;
+  if (key == steps) return 0;
+  std::vector<Dir> d = sphere_directions(half, steps);
+  TRY(ensure(ctx, buf, d.size() * sizeof(Dir)));
+  HIP_TRY(ctx, hipMemcpyAsync(buf.ptr, d.data(), d.size() * sizeof(Dir), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  key = steps;
+  count = (int)d.size();
+  return 0;
+}
+
+int activate(sfsim_atmo_ctx* ctx) {
+  if (!ctx) return SFSIM_ATMO_ERR_ARGUMENT;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return 0;
+}
+
+// ------------------------------------------------------------------ timed launches
+struct Timer {
+  sfsim_atmo_ctx* ctx;
+  bool enabled;
+  int next = 0;
+  int begin(int kind) {
+    if (!enabled) return 0;
+    size_t need = (size_t)next + 2;
+    while (ctx->events.size() < need) {
+      hipEvent_t e;
+      HIP_TRY(ctx, hipEventCreate(&e));
+      ctx->events.push_back(e);
+      ctx->event_kind.push_back(0);
+    }
+    ctx->event_kind[next] = kind;
+    HIP_TRY(ctx, hipEventRecord(ctx->events[next], ctx->stream));
+    return 0;
+  }
+  int end() {
+    if (!enabled) return 0;
+    HIP_TRY(ctx, hipEventRecord(ctx->events[next + 1], ctx->stream));
+    next += 2;
+    return 0;
+  }
+  int collect() {
+    if (!enabled) return 0;
+    for (int k = 0; k < kNumKinds; ++k) {
+      ctx->kind_ms[k] = 0;
+      ctx->kind_launches[k] = 0;
+    }
+    for (int i = 0; i < next; i += 2) {
+      float ms = 0;
+      HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->events[i], ctx->events[i + 1]));
+      ctx->kind_ms[ctx->event_kind[i]] += ms;
+      ctx->kind_launches[ctx->event_kind[i]] += 1;
+    }
+    return 0;
+  }
+};
+
+#define LAUNCH(timer, kind, call)                      \
+  do {                                                 \
+    TRY((timer).begin(kind));                          \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(ctx, #call, e_); \
+    TRY((timer).end());                                \
+  } while (0)
+
+// The whole driver of atmosphere_lut.clj:43-105 on device buffers of the context.
+int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* wc, bool timed) {
+  TRY(validate(ctx, p));
+  if (p->iterations > 0) {
+    TRY(validate_sphere(ctx, p->sphere_steps));
+    TRY(validate_half_sphere(ctx, p->ray_steps));
+    TRY(ensure_dirs(ctx, 0, p->sphere_steps));
+    TRY(ensure_dirs(ctx, 1, p->ray_steps));
+  }
+  const DevParams P = make_dev_params(p);
+  const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  TRY(ensure(ctx, ctx->T, nT * sizeof(float4)));
+  for (int i = 0; i < 2; ++i) {
+    TRY(ensure(ctx, ctx->dE[i], nE * sizeof(float4)));
+    TRY(ensure(ctx, ctx->Eacc[i], nE * sizeof(float4)));
+    TRY(ensure(ctx, ctx->dS[i], nS * sizeof(float4)));
+    TRY(ensure(ctx, ctx->Sacc[i], nS * sizeof(float4)));
+  }
+  TRY(ensure(ctx, ctx->R1, nS * sizeof(float4)));
+  TRY(ensure(ctx, ctx->M1, nS * sizeof(float4)));
+  TRY(ensure(ctx, ctx->dJ, nS * sizeof(float4)));
+  const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
+  for (int i = 0; i < 4; ++i) {
+    TRY(ensure(ctx, ctx->out[i], outc[i] * sizeof(float)));
+    ctx->out_count[i] = outc[i];
+  }
+  ctx->built = false;
+  Timer timer{ctx, timed};
+  auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
+  const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
+  const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
+
+  LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), wc, ctx->stream));
+  LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), wc, ctx->stream));
+  LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream));
+  // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
+  const float4* S_cur = f4(ctx->R1);
+  const float4* E_cur = nullptr;
+  int dE_cur = 0, dS_cur = -1, s_next = 0, e_next = 0;
+  for (int it = 0; it < p->iterations; ++it) {
+    const bool first = it == 0;
+    const float4* dS_tab = first ? f4(ctx->R1) : f4(ctx->dS[dS_cur]);
+    const float4* mie = first ? f4(ctx->M1) : nullptr;
+    // the partials of atmosphere_lut.clj:88-89 bind the previous dS and dE
+    LAUNCH(timer, kPointScatter,
+           launch_point_scatter(P, sdirs, ctx->sphere_n, dS_tab, mie, f4(ctx->dE[dE_cur]), f4(ctx->dJ), wc, ctx->stream));
+    const int dE_new = 1 - dE_cur;
+    LAUNCH(timer, kSurfaceRadiance,
+           launch_surface_radiance(P, hdirs, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
+    const int dS_new = dS_cur < 0 ? 0 : 1 - dS_cur;
+    LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[dS_new]), wc, ctx->stream));
+    dE_cur = dE_new;
+    dS_cur = dS_new;
+    // E <- table(E + dE), S <- table(S + dS) (atmosphere_lut.clj:95-97)
+    LAUNCH(timer, kAccumulate,
+           launch_resample(P, 3, f4(ctx->dE[dE_cur]), E_cur, f4(ctx->Eacc[e_next]), nullptr, 0, ctx->stream));
+    E_cur = f4(ctx->Eacc[e_next]);
+    e_next = 1 - e_next;
+    LAUNCH(timer, kAccumulate,
+           launch_resample(P, 4, S_cur, f4(ctx->dS[dS_cur]), f4(ctx->Sacc[s_next]), nullptr, 0, ctx->stream));
+    S_cur = f4(ctx->Sacc[s_next]);
+    s_next = 1 - s_next;
+  }
+  // final make-lookup-table calls and file layouts (atmosphere_lut.clj:98-105)
+  LAUNCH(timer, kFinal, launch_resample(P, 2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, ctx->stream));
+  if (E_cur) {
+    LAUNCH(timer, kFinal, launch_resample(P, 3, E_cur, nullptr, nullptr, (float*)ctx->out[1].ptr, 0, ctx->stream));
+  } else {
+    HIP_TRY(ctx, hipMemsetAsync(ctx->out[1].ptr, 0, outc[1] * sizeof(float), ctx->stream));
+  }
+  LAUNCH(timer, kFinal, launch_resample(P, 4, S_cur, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
+  LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  TRY(timer.collect());
+  ctx->built = true;
+  return 0;
+}
+
+// ------------------------------------------------------------------ stage entry helpers
+bool finite_floats(const float* v, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+// Upload a host RGB table into float4 staging buffer `slot`.
+int upload_table(sfsim_atmo_ctx* ctx, int slot, const float* host, size_t entries, const float4** dev) {
+  if (!host) {
+    *dev = nullptr;
+    return 0;
+  }
+  if (!finite_floats(host, entries * 3)) return fail(ctx, SFSIM_ATMO_ERR_NONFINITE, "input table holds NaN/Inf");
+  TRY(ensure(ctx, ctx->stage_rgb, entries * 3 * sizeof(float)));
+  TRY(ensure(ctx, ctx->stage_in[slot], entries * sizeof(float4)));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->stage_rgb.ptr, host, entries * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, launch_unpack_rgb((const float*)ctx->stage_rgb.ptr, (float4*)ctx->stage_in[slot].ptr, (long long)entries,
+                                 ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *dev = (const float4*)ctx->stage_in[slot].ptr;
+  return 0;
+}
+
+int download_table(sfsim_atmo_ctx* ctx, const float4* dev, float* host, size_t entries) {
+  if (!host) return 0;
+  TRY(ensure(ctx, ctx->stage_rgb, entries * 3 * sizeof(float)));
+  HIP_TRY(ctx, launch_pack_rgb(dev, (float*)ctx->stage_rgb.ptr, (long long)entries, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(host, ctx->stage_rgb.ptr, entries * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+float4* stage_out(sfsim_atmo_ctx* ctx, size_t entries, int* rc) {
+  *rc = ensure(ctx, ctx->stage_out4, entries * sizeof(float4));
+  return (float4*)ctx->stage_out4.ptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfsim_atmo_create(int device, sfsim_atmo_ctx** out) {
+  if (!out) return SFSIM_ATMO_ERR_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SFSIM_ATMO_ERR_DEVICE;
+  if (device < 0 || device >= n) return SFSIM_ATMO_ERR_ARGUMENT;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SFSIM_ATMO_ERR_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SFSIM_ATMO_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return SFSIM_ATMO_ERR_DEVICE;
+  sfsim_atmo_ctx* ctx = new (std::nothrow) sfsim_atmo_ctx();
+  if (!ctx) return SFSIM_ATMO_ERR_ALLOC;
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return SFSIM_ATMO_ERR_DEVICE;
+  }
+  *out = ctx;
+  return 0;
+}
+
+int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
+  if (!ctx) return SFSIM_ATMO_ERR_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DeviceBuffer* bufs[] = {&ctx->T,      &ctx->dE[0],   &ctx->dE[1],       &ctx->Eacc[0],     &ctx->Eacc[1],
+                          &ctx->R1,     &ctx->M1,      &ctx->dJ,          &ctx->dS[0],       &ctx->dS[1],
+                          &ctx->Sacc[0], &ctx->Sacc[1], &ctx->out[0],     &ctx->out[1],      &ctx->out[2],
+                          &ctx->out[3], &ctx->stage_in[0], &ctx->stage_in[1], &ctx->stage_in[2], &ctx->stage_out4,
+                          &ctx->stage_rgb, &ctx->sphere_dirs, &ctx->half_dirs, &ctx->counters};
+  for (DeviceBuffer* b : bufs) release(*b);
+  for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+const char* sfsim_atmo_last_error(const sfsim_atmo_ctx* ctx) {
+  if (!ctx) return "context is NULL";
+  return ctx->error.c_str();
+}
+
+int sfsim_atmo_transmittance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* out) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (!out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "out is NULL");
+  DevParams P = make_dev_params(p);
+  size_t n = (size_t)count2(p->transmittance_shape);
+  int rc;
+  float4* o = stage_out(ctx, n, &rc);
+  TRY(rc);
+  HIP_TRY(ctx, launch_transmittance(P, o, nullptr, ctx->stream));
+  return download_table(ctx, o, out, n);
+}
+
+int sfsim_atmo_surface_radiance_base_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* out) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (!out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "out is NULL");
+  DevParams P = make_dev_params(p);
+  size_t n = (size_t)count2(p->surface_radiance_shape);
+  int rc;
+  float4* o = stage_out(ctx, n, &rc);
+  TRY(rc);
+  HIP_TRY(ctx, launch_surface_radiance_base(P, o, nullptr, ctx->stream));
+  return download_table(ctx, o, out, n);
+}
+
+int sfsim_atmo_single_scatter_tables(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* rayleigh_out,
+                                     float* mie_out) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  DevParams P = make_dev_params(p);
+  size_t n = (size_t)count4(p->scatter_shape);
+  TRY(ensure(ctx, ctx->R1, n * sizeof(float4)));
+  TRY(ensure(ctx, ctx->M1, n * sizeof(float4)));
+  ctx->built = false;
+  HIP_TRY(ctx, launch_single_scatter(P, (float4*)ctx->R1.ptr, (float4*)ctx->M1.ptr, nullptr, ctx->stream));
+  TRY(download_table(ctx, (const float4*)ctx->R1.ptr, rayleigh_out, n));
+  return download_table(ctx, (const float4*)ctx->M1.ptr, mie_out, n);
+}
+
+int sfsim_atmo_point_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const float* dS,
+                                   const float* mie_strength, const float* dE, float* out) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  TRY(validate_sphere(ctx, p->sphere_steps));
+  if (!dS || !dE || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dS, dE and out must not be NULL");
+  TRY(ensure_dirs(ctx, 0, p->sphere_steps));
+  DevParams P = make_dev_params(p);
+  size_t n = (size_t)count4(p->scatter_shape), ne = (size_t)count2(p->surface_radiance_shape);
+  const float4 *ddS, *dM, *ddE;
+  TRY(upload_table(ctx, 0, dS, n, &ddS));
+  TRY(upload_table(ctx, 1, mie_strength, n, &dM));
+  TRY(upload_table(ctx, 2, dE, ne, &ddE));
+  int rc;
+  float4* o = stage_out(ctx, n, &rc);
+  TRY(rc);
+  HIP_TRY(ctx, launch_point_scatter(P, (const Dir*)ctx->sphere_dirs.ptr, ctx->sphere_n, ddS, dM, ddE, o, nullptr,
+                                    ctx->stream));
+  return download_table(ctx, o, out, n);
+}
+
+int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const float* dJ, float* out) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (!dJ || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dJ and out must not be NULL");
+  DevParams P = make_dev_params(p);
+  size_t n = (size_t)count4(p->scatter_shape);
+  const float4* ddJ;
+  TRY(upload_table(ctx, 0, dJ, n, &ddJ));
+  int rc;
+  float4* o = stage_out(ctx, n, &rc);
+  TRY(rc);
+  HIP_TRY(ctx, launch_ray_scatter(P, ddJ, o, nullptr, ctx->stream));
+  return download_table(ctx, o, out, n);
+}
+
+int sfsim_atmo_surface_radiance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const float* dS,
+                                      const float* mie_strength, float* out) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  TRY(validate_half_sphere(ctx, p->ray_steps));
+  if (!dS || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dS and out must not be NULL");
+  TRY(ensure_dirs(ctx, 1, p->ray_steps));
+  DevParams P = make_dev_params(p);
+  size_t n = (size_t)count4(p->scatter_shape), ne = (size_t)count2(p->surface_radiance_shape);
+  const float4 *ddS, *dM;
+  TRY(upload_table(ctx, 0, dS, n, &ddS));
+  TRY(upload_table(ctx, 1, mie_strength, n, &dM));
+  int rc;
+  float4* o = stage_out(ctx, ne, &rc);
+  TRY(rc);
+  HIP_TRY(ctx, launch_surface_radiance(P, (const Dir*)ctx->half_dirs.ptr, ctx->half_n, ddS, dM, o, nullptr,
+                                       ctx->stream));
+  return download_table(ctx, o, out, ne);
+}
+
+int sfsim_atmo_resample_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int space, const float* a,
+                              const float* b, float* out) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (space != 2 && space != 3 && space != 4) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "space must be 2, 3 or 4");
+  if (!a || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "a and out must not be NULL");
+  DevParams P = make_dev_params(p);
+  size_t n = space == 4   ? (size_t)count4(p->scatter_shape)
+             : space == 2 ? (size_t)count2(p->transmittance_shape)
+                          : (size_t)count2(p->surface_radiance_shape);
+  const float4 *da, *db;
+  TRY(upload_table(ctx, 0, a, n, &da));
+  TRY(upload_table(ctx, 1, b, n, &db));
+  int rc;
+  float4* o = stage_out(ctx, n, &rc);
+  TRY(rc);
+  HIP_TRY(ctx, launch_resample(P, space, da, db, o, nullptr, 0, ctx->stream));
+  return download_table(ctx, o, out, n);
+}
+
+int sfsim_atmo_build(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  TRY(activate(ctx));
+  return run_pipeline(ctx, p, nullptr, true);
+}
+
+int sfsim_atmo_fetch(sfsim_atmo_ctx* ctx, int which, float* out, size_t count) {
+  TRY(activate(ctx));
+  if (which < 0 || which > 3 || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "which must be 0..3 and out non-NULL");
+  if (!ctx->built) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "no completed build on this context");
+  if (count != ctx->out_count[which]) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "count does not match the table size");
+  HIP_TRY(ctx, hipMemcpyAsync(out, ctx->out[which].ptr, count * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sfsim_atmo_generate_luts(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* transmittance,
+                             float* surface_radiance, float* ray_scatter, float* mie_strength) {
+  TRY(activate(ctx));
+  TRY(run_pipeline(ctx, p, nullptr, false));
+  float* outs[4] = {transmittance, surface_radiance, ray_scatter, mie_strength};
+  for (int i = 0; i < 4; ++i)
+    if (outs[i]) TRY(sfsim_atmo_fetch(ctx, i, outs[i], ctx->out_count[i]));
+  return 0;
+}
+
+int sfsim_atmo_stage_timings(const sfsim_atmo_ctx* ctx, double* ms, int* launches, int n) {
+  if (!ctx) return SFSIM_ATMO_ERR_ARGUMENT;
+  for (int k = 0; k < n && k < kNumKinds; ++k) {
+    if (ms) ms[k] = ctx->kind_ms[k];
+    if (launches) launches[k] = ctx->kind_launches[k];
+  }
+  return kNumKinds;
+}
+
+const char* sfsim_atmo_stage_name(int kind) {
+  if (kind < 0 || kind >= kNumKinds) return "";
+  return kKindNames[kind];
+}
+
+int sfsim_atmo_count_work(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, unsigned long long* counts) {
+  TRY(activate(ctx));
+  if (!counts) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "counts is NULL");
+  TRY(ensure(ctx, ctx->counters, sizeof(WorkCounters)));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->counters.ptr, 0, sizeof(WorkCounters), ctx->stream));
+  TRY(run_pipeline(ctx, p, (WorkCounters*)ctx->counters.ptr, false));
+  WorkCounters wc;
+  HIP_TRY(ctx, hipMemcpy(&wc, ctx->counters.ptr, sizeof(wc), hipMemcpyDeviceToHost));
+  counts[0] = wc.single_samples;
+  counts[1] = wc.samples;
+  counts[2] = wc.lookups4;
+  counts[3] = wc.lookups2;
+  return 0;
+}
+
+int sfsim_atmo_write_scatter(const char* path, const float* data, size_t count) {
+  if (!path || (!data && count)) return SFSIM_ATMO_ERR_ARGUMENT;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return SFSIM_ATMO_ERR_IO;
+  // little-endian float32 (util.clj:227-240); gfx950 hosts are little-endian x86-64
+  size_t w = count ? std::fwrite(data, sizeof(float), count, f) : 0;
+  int rc = std::fclose(f);
+  return (w == count && rc == 0) ? 0 : SFSIM_ATMO_ERR_IO;
+}
+
+int sfsim_atmo_read_scatter(const char* path, float* data, size_t count) {
+  if (!path || (!data && count)) return SFSIM_ATMO_ERR_ARGUMENT;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return SFSIM_ATMO_ERR_IO;
+  size_t r = count ? std::fread(data, sizeof(float), count, f) : 0;
+  int extra = std::fgetc(f);
+  std::fclose(f);
+  return (r == count && extra == EOF) ? 0 : SFSIM_ATMO_ERR_IO;
+}
+
+int sfsim_atmo_sphere_directions(int half, int steps) {
+  int theta_steps = half ? (steps >> 2) : (steps >> 1);
+  if (theta_steps < 1) return 0;
+  double theta_range = half ? M_PI / 2 : M_PI;
+  int total = 0;
+  for (int i = 0; i < theta_steps; ++i)
+    total += (int)std::ceil(std::sin(theta_range * ((0.5 + i) / theta_steps)) * steps);
+  return total;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Parameters of one table build as the kernels see them (host fills, passed by value).
+#pragma once
+
+#include "geometry.hpp"
+
+namespace sfsim {
+
+constexpr int kMaxSteps = 256;   // ray_steps limit (LDS arrays of the per-ray kernels)
+constexpr int kMaxSphereDirs = 2048;
+
+struct DevParams {
+  Geo g;
+  // extinction (scatter-base / scatter-quotient) and scattering (scatter-base) per component, FP32
+  float ext_mie[3], ext_ray[3];
+  float sca_mie[3], sca_ray[3];
+  // exp(-h / scale) = exp2(h * k): k = -log2(e) / scale
+  float k_mie, k_ray;
+  double inv_scale_mie, inv_scale_ray;
+  float g_mie, g_ray;
+  float brightness_over_pi[3];
+  float intensity[3];
+  int nT[2];  // transmittance shape  [height elevation]
+  int nE[2];  // surface radiance shape [height sun-elevation]
+  int nS[4];  // scatter shape [height elevation sun-elevation heading]
+  int ray_steps;
+  // optical depth of a segment whose lowest point lies strictly inside it and below this altitude
+  // is accumulated in FP64 (near-horizon / grazing rays); everything else in FP32 + compensation
+  double graze_height;
+};
+
+// Quadrature direction (FP64 unit vector in the frame of a grid point on +x) and its weight.
+struct Dir {
+  double x, y, z, w;
+};
+
+// Optional device counters of the work the kernels do (instrumented builds only).
+struct WorkCounters {
+  unsigned long long single_samples;  // extinction samples in the single-scatter kernel
+  unsigned long long samples;         // extinction samples, all kernels
+  unsigned long long lookups4;        // 4D multilinear lookups
+  unsigned long long lookups2;        // 2D multilinear lookups
+};
+
+}  // namespace sfsim

@@ -1,0 +1,636 @@
+// HIP kernels (gfx950) of the atmosphere lookup-table precompute.
+//
+// Mapping: one table entry per lane.  The 4D kernels run one workgroup per view ray
+// (height index, elevation index) of the scatter grid: the ray geometry, the per-sample altitudes
+// and the view-ray transmittance T(x, p_i) depend only on the ray, so the workgroup computes them
+// once into LDS and every lane (sun-elevation, heading entry) of that ray reuses them.
+// Quadratures follow the reference exactly (midpoint nodes of ray.clj:19-30, latitude rings of
+// sphere.clj:62-105, O(n^2) transmittance of atmosphere.clj:192-200); only loop-invariant work is
+// shared.  Geometry (intersections, the sun direction at each sample) is FP64; the extinction
+// sums along rays are FP32 with compensated accumulation, switching to FP64 for segments whose
+// lowest point is interior and below DevParams::graze_height.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace sfsim {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
+  float y = x - comp;
+  float t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+}
+
+// atmosphere.clj:56-61 in FP32
+__device__ __forceinline__ float phase_f(float g, float mu) {
+  float g2 = g * g;
+  float x = (1.0f + g2) - 2.0f * g * mu;
+  return (3.0f * (1.0f - g2) * (1.0f + mu * mu)) / (8.0f * 3.14159265358979f * (2.0f + g2) * (x * sqrtf(x)));
+}
+
+// Sums over midpoint nodes t_j = (j + 1/2) dt, j < n, of exp(-h_j / H_mie) and exp(-h_j / H_rayleigh)
+// along a ray from altitude h0 (c0 = h0 (2R + h0)) with b = 2 dot(point, unit direction):
+// h(t) = (c0 + t (b + t)) / (R + sqrt(R^2 + c0 + t (b + t))).  8-node partial sums, Kahan across them.
+__device__ float2 density_sums_f32(const DevParams& P, float c0, float b, float dt, int n) {
+  const float R = (float)P.g.R;
+  const float R2 = (float)(P.g.R * P.g.R);
+  const float half = 0.5f * dt;
+  const float km = P.k_mie, kr = P.k_ray;
+  float sm = 0.f, sr = 0.f, cm = 0.f, cr = 0.f;
+  int j = 0;
+  for (; j + 8 <= n; j += 8) {
+    float pm = 0.f, pr = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      float t = fmaf((float)(j + q), dt, half);
+      float u = fmaf(t, t + b, c0);
+      float h = u * frcp(R + fsqrt(R2 + u));
+      pm += fexp2(h * km);
+      pr += fexp2(h * kr);
+    }
+    kahan_add(sm, cm, pm);
+    kahan_add(sr, cr, pr);
+  }
+  if (j < n) {
+    float pm = 0.f, pr = 0.f;
+    for (; j < n; ++j) {
+      float t = fmaf((float)j, dt, half);
+      float u = fmaf(t, t + b, c0);
+      float h = u * frcp(R + fsqrt(R2 + u));
+      pm += fexp2(h * km);
+      pr += fexp2(h * kr);
+    }
+    kahan_add(sm, cm, pm);
+    kahan_add(sr, cr, pr);
+  }
+  return make_float2(sm, sr);
+}
+
+__device__ double2 density_sums_f64(const DevParams& P, double h0, double pd0, double dt, int n) {
+  const double R = P.g.R;
+  const double c0 = h0 * (2.0 * R + h0), b = 2.0 * pd0;
+  double sm = 0.0, sr = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double t = (j + 0.5) * dt;
+    double u = c0 + t * (b + t);
+    double h = u / (R + sqrt(R * R + u));
+    sm += exp(-h * P.inv_scale_mie);
+    sr += exp(-h * P.inv_scale_ray);
+  }
+  return make_double2(sm, sr);
+}
+
+// True when the lowest point of the segment [0, D] lies strictly inside it and below graze_height.
+__device__ __forceinline__ bool grazing(const DevParams& P, double h0, double pd0, double D) {
+  double tp = -pd0;
+  if (!(tp > 0.0 && tp < D)) return false;
+  double u = h0 * (2.0 * P.g.R + h0) - pd0 * pd0;
+  double hp = u / (P.g.R + sqrt(P.g.R * P.g.R + u));
+  return hp < P.graze_height;
+}
+
+// Transmittance of the segment of length D from altitude h0 along a unit direction with
+// dot(point, direction) = pd0, n midpoint samples (atmosphere.clj:114-125).
+__device__ float3 segment_transmittance(const DevParams& P, double h0, double pd0, double D, int n) {
+  if (!(D > 0.0)) return make_float3(1.f, 1.f, 1.f);
+  double dtd = D / n;
+  float sm, sr;
+  if (grazing(P, h0, pd0, D)) {
+    double2 s = density_sums_f64(P, h0, pd0, dtd, n);
+    sm = (float)(s.x * dtd);
+    sr = (float)(s.y * dtd);
+  } else {
+    float dt = (float)dtd;
+    float2 s = density_sums_f32(P, (float)(h0 * (2.0 * P.g.R + h0)), (float)(2.0 * pd0), dt, n);
+    sm = s.x * dt;
+    sr = s.y * dt;
+  }
+  return make_float3(expf(-(P.ext_mie[0] * sm + P.ext_ray[0] * sr)), expf(-(P.ext_mie[1] * sm + P.ext_ray[1] * sr)),
+                     expf(-(P.ext_mie[2] * sm + P.ext_ray[2] * sr)));
+}
+
+// ------------------------------------------------------------------ multilinear lookups
+// interpolate.clj:75-98: per dimension i = clip(c), u = floor(i), v = clip(u + 1), s = i - u.
+struct Lerp {
+  int i0, i1;
+  float s;
+};
+
+__device__ __forceinline__ Lerp make_lerp(float c, int N) {
+  float i = fminf(fmaxf(c, 0.0f), (float)(N - 1));
+  float u = floorf(i);
+  Lerp L;
+  L.i0 = (int)u;
+  L.i1 = min(L.i0 + 1, N - 1);
+  L.s = i - u;
+  return L;
+}
+
+__device__ __forceinline__ float3 mix3(float3 a, float3 b, float s) {
+  float r = 1.0f - s;
+  return make_float3(a.x * r + b.x * s, a.y * r + b.y * s, a.z * r + b.z * s);
+}
+
+__device__ __forceinline__ float3 ld3(const float4* __restrict__ T, long long i) {
+  float4 v = T[i];
+  return make_float3(v.x, v.y, v.z);
+}
+
+// 2D table [N0][N1]
+__device__ __forceinline__ float3 lookup2(const float4* __restrict__ T, int N1, Lerp a, Lerp b) {
+  float3 u = mix3(ld3(T, (long long)a.i0 * N1 + b.i0), ld3(T, (long long)a.i0 * N1 + b.i1), b.s);
+  float3 v = mix3(ld3(T, (long long)a.i1 * N1 + b.i0), ld3(T, (long long)a.i1 * N1 + b.i1), b.s);
+  return mix3(u, v, a.s);
+}
+
+// 4D table [N0][N1][N2][N3], nested like interpolate-value (outer dimension mixed last)
+__device__ __forceinline__ float3 lookup4(const float4* __restrict__ T, int N1, int N2, int N3, Lerp a, Lerp b, Lerp c,
+                                          Lerp d) {
+  float3 r[2];
+#pragma unroll
+  for (int ia = 0; ia < 2; ++ia) {
+    long long ba = (long long)(ia ? a.i1 : a.i0) * N1;
+    float3 rb[2];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      long long bb = (ba + (ib ? b.i1 : b.i0)) * N2;
+      long long c0 = (bb + c.i0) * N3, c1 = (bb + c.i1) * N3;
+      float3 u = mix3(ld3(T, c0 + d.i0), ld3(T, c0 + d.i1), d.s);
+      float3 v = mix3(ld3(T, c1 + d.i0), ld3(T, c1 + d.i1), d.s);
+      rb[ib] = mix3(u, v, c.s);
+    }
+    r[ia] = mix3(rb[0], rb[1], b.s);
+  }
+  return mix3(r[0], r[1], a.s);
+}
+
+__device__ __forceinline__ float sun_elevation_coord(int size, float sin_elevation) {
+  // atmosphere.clj:322-326; 1 / (1 - exp(-3.6))
+  const float inv_den = 1.0280583777619436f;
+  return (size - 1) * fmaxf(0.0f, (1.0f - __expf(-3.0f * sin_elevation - 0.6f)) * inv_den);
+}
+
+__device__ __forceinline__ void count(WorkCounters* wc, unsigned long long single, unsigned long long samples,
+                                      unsigned long long l4, unsigned long long l2) {
+  if (wc) {
+    if (single) atomicAdd(&wc->single_samples, single);
+    if (samples) atomicAdd(&wc->samples, samples);
+    if (l4) atomicAdd(&wc->lookups4, l4);
+    if (l2) atomicAdd(&wc->lookups2, l2);
+  }
+}
+
+// ------------------------------------------------------------------------------ kernels
+// T table: atmosphere_lut.clj:74 over transmittance-space (atmosphere.clj:291-319)
+__global__ void __launch_bounds__(kBlock) transmittance_kernel(const DevParams P, float4* __restrict__ out,
+                                                               WorkCounters* wc) {
+  const int NH = P.nT[0], NE = P.nT[1];
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NH * NE) return;
+  int ih = idx / NE, ie = idx - ih * NE;
+  double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
+  double se, ce;
+  bool above;
+  index_to_elevation(P.g, NE, r0, ie, se, ce, above);
+  double pd0 = r0 * se;
+  double D = above ? top_exit_distance(P.g, h0, pd0) : ground_distance(P.g, h0, pd0);
+  float3 T = segment_transmittance(P, h0, pd0, D, P.ray_steps);
+  out[idx] = make_float4(T.x, T.y, T.z, 0.f);
+  count(wc, 0, P.ray_steps, 0, 0);
+}
+
+// dE0 table: surface-radiance-base (atmosphere.clj:131-137) over surface-radiance-space
+__global__ void __launch_bounds__(kBlock) surface_radiance_base_kernel(const DevParams P, float4* __restrict__ out,
+                                                                       WorkCounters* wc) {
+  const int NH = P.nE[0], NL = P.nE[1];
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NH * NL) return;
+  int ih = idx / NL, il = idx - ih * NL;
+  double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
+  double s = index_to_sin_sun_elevation(NL, il);
+  double pd0 = r0 * s;  // dot(x, l) with l = (s, c, 0)
+  double D = top_exit_distance(P.g, h0, pd0);
+  float3 T = segment_transmittance(P, h0, pd0, D, P.ray_steps);
+  float cosf_ = (float)fmax(0.0, s);  // max(0, normal . l), normal = (1 0 0)
+  out[idx] = make_float4(T.x * P.intensity[0] * cosf_, T.y * P.intensity[1] * cosf_, T.z * P.intensity[2] * cosf_, 0.f);
+  count(wc, 0, P.ray_steps, 0, 0);
+}
+
+// Per-ray setup shared by the 4D kernels: grid point x = (r0 0 0), view v = (se ce 0), ray length D.
+struct Ray {
+  double r0, h0, se, ce, pd0, D;
+  bool above;
+};
+
+__device__ __forceinline__ Ray make_ray(const DevParams& P, int ih, int ie) {
+  Ray r;
+  r.r0 = index_to_radius(P.g, P.nS[0], ih);
+  r.h0 = index_to_altitude(P.g, P.nS[0], ih);
+  index_to_elevation(P.g, P.nS[1], r.r0, ie, r.se, r.ce, r.above);
+  r.pd0 = r.r0 * r.se;
+  r.D = r.above ? top_exit_distance(P.g, r.h0, r.pd0) : ground_distance(P.g, r.h0, r.pd0);
+  return r;
+}
+
+// R1 (Rayleigh with phase) and M1 (Mie strength) in one pass: atmosphere_lut.clj:70-78,
+// atmosphere.clj:154-200.  One workgroup per view ray; lanes over (sun elevation, heading).
+__global__ void __launch_bounds__(kBlock) single_scatter_kernel(const DevParams P, float4* __restrict__ R1,
+                                                                float4* __restrict__ M1, WorkCounters* wc) {
+  __shared__ double s_t[kMaxSteps];
+  __shared__ double s_h[kMaxSteps];
+  __shared__ float4 s_w[kMaxSteps];    // T(x, p_i) rgb, w: exp(-h_i / H_mie)
+  __shared__ float s_bray[kMaxSteps];  // exp(-h_i / H_rayleigh)
+  const int NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int n = P.ray_steps;
+  const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
+  const Ray ray = make_ray(P, ih, ie);
+  const double dt = ray.D / n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double t = (i + 0.5) * dt;
+    double h = altitude_along(P.g, ray.h0, ray.pd0, t);
+    s_t[i] = t;
+    s_h[i] = h;
+    float3 T = segment_transmittance(P, ray.h0, ray.pd0, t, n);
+    s_w[i] = make_float4(T.x, T.y, T.z, (float)exp(-h * P.inv_scale_mie));
+    s_bray[i] = (float)exp(-h * P.inv_scale_ray);
+  }
+  __syncthreads();
+  unsigned long long samples = 0;
+  const long long base = (long long)blockIdx.x * NL * NA;
+  for (int k = threadIdx.x; k < NL * NA; k += blockDim.x) {
+    int is = k / NA, ia = k - is * NA;
+    double sin_s = index_to_sin_sun_elevation(NL, is);
+    double l0, l1, l2;
+    index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
+    double vl = ray.se * l0 + ray.ce * l1;
+    double xl = ray.r0 * l0;
+    float3 accR = make_float3(0.f, 0.f, 0.f), accM = make_float3(0.f, 0.f, 0.f);
+    for (int i = 0; i < n; ++i) {
+      double h = s_h[i];
+      double pl = xl + s_t[i] * vl;  // dot(p_i, l)
+      if (!is_above_horizon(P.g, h, pl)) continue;  // no direct sun (atmosphere.clj:154-160)
+      double Ds = top_exit_distance(P.g, h, pl);
+      float3 Ts = segment_transmittance(P, h, pl, Ds, n);
+      samples += n;
+      float4 w = s_w[i];
+      float br = s_bray[i];
+      float3 q = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
+      accR.x += q.x * br; accR.y += q.y * br; accR.z += q.z * br;
+      accM.x += q.x * w.w; accM.y += q.y * w.w; accM.z += q.z * w.w;
+    }
+    float a = (float)dt;
+    float pr = phase_f(P.g_ray, (float)vl) * a;
+    R1[base + k] = make_float4(accR.x * P.sca_ray[0] * P.intensity[0] * pr, accR.y * P.sca_ray[1] * P.intensity[1] * pr,
+                               accR.z * P.sca_ray[2] * P.intensity[2] * pr, 0.f);
+    M1[base + k] = make_float4(accM.x * P.sca_mie[0] * P.intensity[0] * a, accM.y * P.sca_mie[1] * P.intensity[1] * a,
+                               accM.z * P.sca_mie[2] * P.intensity[2] * a, 0.f);
+  }
+  unsigned long long view = (threadIdx.x < n) ? (unsigned long long)n * ((n - threadIdx.x + blockDim.x - 1) / blockDim.x) : 0;
+  count(wc, samples, samples + view, 0, 0);
+}
+
+// dS' = ray-scatter of an interpolated dJ table (atmosphere_lut.clj:92,94; atmosphere.clj:192-200)
+__global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
+                                                             float4* __restrict__ out, WorkCounters* wc) {
+  __shared__ float4 s_w[kMaxSteps];  // T(x, p_i) rgb, w: height coordinate of p_i
+  __shared__ float2 s_g[kMaxSteps];  // x: elevation coordinate of v at p_i, y: t_i
+  __shared__ float s_rinv[kMaxSteps];
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int n = P.ray_steps;
+  const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
+  const Ray ray = make_ray(P, ih, ie);
+  const double dt = ray.D / n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double t = (i + 0.5) * dt;
+    double h = altitude_along(P.g, ray.h0, ray.pd0, t);
+    double r = P.g.R + h;
+    float3 T = segment_transmittance(P, ray.h0, ray.pd0, t, n);
+    s_w[i] = make_float4(T.x, T.y, T.z, (float)altitude_to_index(P.g, NH, h));
+    s_g[i] = make_float2((float)elevation_to_index(P.g, NE, r, ray.pd0 + t, ray.above), (float)t);
+    s_rinv[i] = (float)(1.0 / r);
+  }
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * NL * NA;
+  unsigned long long lookups = 0;
+  for (int k = threadIdx.x; k < NL * NA; k += blockDim.x) {
+    int is = k / NA, ia = k - is * NA;
+    double sin_s = index_to_sin_sun_elevation(NL, is);
+    double l0, l1, l2;
+    index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
+    float vl = (float)(ray.se * l0 + ray.ce * l1);
+    float xl = (float)(ray.r0 * l0);
+    Lerp La = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    for (int i = 0; i < n; ++i) {
+      float4 w = s_w[i];
+      float2 gi = s_g[i];
+      float sin_e = fmaf(gi.y, vl, xl) * s_rinv[i];
+      float3 v = lookup4(dJ, NE, NL, NA, make_lerp(w.w, NH), make_lerp(gi.x, NE),
+                         make_lerp(sun_elevation_coord(NL, sin_e), NL), La);
+      acc.x += w.x * v.x; acc.y += w.y * v.y; acc.z += w.z * v.z;
+    }
+    lookups += n;
+    float a = (float)dt;
+    out[base + k] = make_float4(acc.x * a, acc.y * a, acc.z * a, 0.f);
+  }
+  unsigned long long view = (threadIdx.x < n) ? (unsigned long long)n * ((n - threadIdx.x + blockDim.x - 1) / blockDim.x) : 0;
+  count(wc, 0, view, lookups, 0);
+}
+
+// dJ = point-scatter over the sphere quadrature (atmosphere_lut.clj:88,91; atmosphere.clj:203-222).
+// One workgroup per view ray (x, v); per-direction data in LDS, lanes over (sun elevation, heading).
+struct DirData {
+  float4 omega;   // direction xyz, w: elevation coordinate of omega at x (flag: not surface)
+  float4 ground;  // T(x, P) * brightness / pi (0 for sky directions), w: 1 for surface directions
+  float4 phat;    // P / |P|, w: height coordinate of P in the surface-radiance table
+  float4 weight;  // sum_c beta_c(h(x)) phase_c(v . omega) times the quadrature weight, rgb
+};
+
+__global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
+                                                               int ndirs, const float4* __restrict__ dS,
+                                                               const float4* __restrict__ mie,
+                                                               const float4* __restrict__ dE, float4* __restrict__ out,
+                                                               WorkCounters* wc) {
+  extern __shared__ DirData s_dir[];
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int n = P.ray_steps;
+  const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
+  const double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
+  double se, ce;
+  bool above_unused;
+  index_to_elevation(P.g, NE, r0, ie, se, ce, above_unused);
+  const float bm = (float)exp(-h0 * P.inv_scale_mie), br = (float)exp(-h0 * P.inv_scale_ray);
+  unsigned long long samples = 0;
+  for (int j = threadIdx.x; j < ndirs; j += blockDim.x) {
+    Dir d = dirs[j];
+    double pd = r0 * d.x;  // dot(x, omega)
+    bool above = is_above_horizon(P.g, h0, pd);
+    double D = above ? top_exit_distance(P.g, h0, pd) : ground_distance(P.g, h0, pd);
+    double hP = altitude_along(P.g, h0, pd, D);
+    bool surface = 2.0 * hP < P.g.H;  // atmosphere.clj:88-92
+    DirData dd;
+    dd.omega = make_float4((float)d.x, (float)d.y, (float)d.z, (float)elevation_to_index(P.g, NE, r0, pd, !surface));
+    if (surface) {
+      float3 T = segment_transmittance(P, h0, pd, D, n);
+      samples += n;
+      dd.ground = make_float4(T.x * P.brightness_over_pi[0], T.y * P.brightness_over_pi[1],
+                              T.z * P.brightness_over_pi[2], 1.f);
+      double px = r0 + d.x * D, py = d.y * D, pz = d.z * D;
+      double pn = 1.0 / sqrt(px * px + py * py + pz * pz);
+      dd.phat = make_float4((float)(px * pn), (float)(py * pn), (float)(pz * pn),
+                            (float)altitude_to_index(P.g, P.nE[0], hP));
+    } else {
+      dd.ground = make_float4(0.f, 0.f, 0.f, 0.f);
+      dd.phat = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float vw = (float)(se * d.x + ce * d.y);
+    float pm = phase_f(P.g_mie, vw) * bm, pr = phase_f(P.g_ray, vw) * br;
+    float w = (float)d.w;
+    dd.weight = make_float4((P.sca_mie[0] * pm + P.sca_ray[0] * pr) * w, (P.sca_mie[1] * pm + P.sca_ray[1] * pr) * w,
+                            (P.sca_mie[2] * pm + P.sca_ray[2] * pr) * w, 0.f);
+    s_dir[j] = dd;
+  }
+  __syncthreads();
+  const Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
+  const long long base = (long long)blockIdx.x * NL * NA;
+  unsigned long long l4 = 0, l2 = 0;
+  for (int k = threadIdx.x; k < NL * NA; k += blockDim.x) {
+    int is = k / NA, ia = k - is * NA;
+    double sin_s = index_to_sin_sun_elevation(NL, is);
+    double l0, l1, l2d;
+    index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2d);
+    const float lx = (float)l0, ly = (float)l1, lz = (float)l2d;
+    const Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    for (int j = 0; j < ndirs; ++j) {
+      const DirData& dd = s_dir[j];
+      float ol = dd.omega.x * lx + dd.omega.y * ly + dd.omega.z * lz;
+      Lerp Le = make_lerp(dd.omega.w, NE);
+      Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
+      float3 v = lookup4(dS, NE, NL, NA, Lh, Le, Ls, La);
+      if (mie) {
+        float3 m = lookup4(mie, NE, NL, NA, Lh, Le, Ls, La);
+        float ph = phase_f(P.g_mie, ol);
+        v.x += m.x * ph; v.y += m.y * ph; v.z += m.z * ph;
+      }
+      if (dd.ground.w != 0.f) {
+        float spl = dd.phat.x * lx + dd.phat.y * ly + dd.phat.z * lz;
+        float3 e = lookup2(dE, P.nE[1], make_lerp(dd.phat.w, P.nE[0]), make_lerp(sun_elevation_coord(P.nE[1], spl), P.nE[1]));
+        v.x += dd.ground.x * e.x; v.y += dd.ground.y * e.y; v.z += dd.ground.z * e.z;
+        ++l2;
+      }
+      acc.x += dd.weight.x * v.x; acc.y += dd.weight.y * v.y; acc.z += dd.weight.z * v.z;
+    }
+    l4 += (unsigned long long)ndirs * (mie ? 2 : 1);
+    out[base + k] = make_float4(acc.x, acc.y, acc.z, 0.f);
+  }
+  count(wc, 0, samples, l4, l2);
+}
+
+// dE' = surface-radiance of dS (atmosphere_lut.clj:89,93; atmosphere.clj:225-230): one workgroup per
+// entry of the surface-radiance table, lanes over half-sphere directions, workgroup reduction.
+__global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P, const Dir* __restrict__ dirs,
+                                                                  int ndirs, const float4* __restrict__ dS,
+                                                                  const float4* __restrict__ mie,
+                                                                  float4* __restrict__ out, WorkCounters* wc) {
+  __shared__ float3 s_red[kBlock / 64];
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int ih = blockIdx.x / P.nE[1], il = blockIdx.x - ih * P.nE[1];
+  const double r0 = index_to_radius(P.g, P.nE[0], ih);
+  const double s = index_to_sin_sun_elevation(P.nE[1], il);
+  const double c = sqrt(fmax(0.0, 1 - s * s));
+  const float lx = (float)s, ly = (float)c;
+  const Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
+  const Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * s) / r0), NL);
+  float3 acc = make_float3(0.f, 0.f, 0.f);
+  for (int j = threadIdx.x; j < ndirs; j += blockDim.x) {
+    Dir d = dirs[j];
+    float ol = (float)d.x * lx + (float)d.y * ly;
+    Lerp Le = make_lerp((float)elevation_to_index(P.g, NE, r0, r0 * d.x, true), NE);
+    Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
+    float3 v = lookup4(dS, NE, NL, NA, Lh, Le, Ls, La);
+    if (mie) {
+      float3 m = lookup4(mie, NE, NL, NA, Lh, Le, Ls, La);
+      float ph = phase_f(P.g_mie, ol);
+      v.x += m.x * ph; v.y += m.y * ph; v.z += m.z * ph;
+    }
+    float w = (float)(d.x * d.w);  // (omega . normal) * quadrature weight, normal = (1 0 0)
+    acc.x += v.x * w; acc.y += v.y * w; acc.z += v.z * w;
+  }
+  // wavefront reduction, then across the workgroup's wavefronts
+  for (int off = 32; off > 0; off >>= 1) {
+    acc.x += __shfl_down(acc.x, off, 64);
+    acc.y += __shfl_down(acc.y, off, 64);
+    acc.z += __shfl_down(acc.z, off, 64);
+  }
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_red[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float3 t = s_red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      t.x += s_red[w].x; t.y += s_red[w].y; t.z += s_red[w].z;
+    }
+    out[blockIdx.x] = make_float4(t.x, t.y, t.z, 0.f);
+  }
+  if (threadIdx.x == 0) count(wc, 0, 0, (unsigned long long)ndirs * (mie ? 2 : 1), 0);
+}
+
+// f(backward(i)) re-sampling of interpolated tables (interpolate.clj:54-72 with interpolation-table).
+__global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int space, const float4* __restrict__ a,
+                                                          const float4* __restrict__ b, float4* __restrict__ out4,
+                                                          float* __restrict__ out_rgb, int tiled) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float3 v;
+  long long dst;
+  if (space == 4) {
+    const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+    if (idx >= (long long)NH * NE * NL * NA) return;
+    int ia = (int)(idx % NA);
+    long long q = idx / NA;
+    int is = (int)(q % NL);
+    q /= NL;
+    int ie = (int)(q % NE);
+    int ih = (int)(q / NE);
+    double r0 = index_to_radius(P.g, NH, ih);
+    double se, ce;
+    bool above;
+    index_to_elevation(P.g, NE, r0, ie, se, ce, above);
+    double sin_s = index_to_sin_sun_elevation(NL, is);
+    double l0, l1, l2;
+    index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2);
+    Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
+    Lerp Le = make_lerp((float)elevation_to_index(P.g, NE, r0, r0 * se, above), NE);
+    Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
+    Lerp La = make_lerp((float)sun_angle_to_index(NA, se * l0 + ce * l1), NA);
+    v = lookup4(a, NE, NL, NA, Lh, Le, Ls, La);
+    if (b) {
+      float3 w = lookup4(b, NE, NL, NA, Lh, Le, Ls, La);
+      v.x += w.x; v.y += w.y; v.z += w.z;
+    }
+    // image.clj:299-312: row h * NL + s, column e * NA + a
+    dst = tiled ? ((long long)(ih * NL + is) * (NE * NA) + (long long)ie * NA + ia) : idx;
+  } else {
+    const int N0 = space == 2 ? P.nT[0] : P.nE[0], N1 = space == 2 ? P.nT[1] : P.nE[1];
+    if (idx >= (long long)N0 * N1) return;
+    int i0 = (int)(idx / N1), i1 = (int)(idx % N1);
+    double r0 = index_to_radius(P.g, N0, i0);
+    double c1;
+    if (space == 2) {
+      double se, ce;
+      bool above;
+      index_to_elevation(P.g, N1, r0, i1, se, ce, above);
+      c1 = elevation_to_index(P.g, N1, r0, r0 * se, above);
+    } else {
+      double s = index_to_sin_sun_elevation(N1, i1);
+      c1 = sun_elevation_to_index(N1, (r0 * s) / r0);
+    }
+    Lerp L0 = make_lerp((float)height_to_index(P.g, N0, r0), N0);
+    Lerp L1 = make_lerp((float)c1, N1);
+    v = lookup2(a, N1, L0, L1);
+    if (b) {
+      float3 w = lookup2(b, N1, L0, L1);
+      v.x += w.x; v.y += w.y; v.z += w.z;
+    }
+    dst = idx;
+  }
+  if (out4) out4[idx] = make_float4(v.x, v.y, v.z, 0.f);
+  if (out_rgb) {
+    out_rgb[dst * 3 + 0] = v.x;
+    out_rgb[dst * 3 + 1] = v.y;
+    out_rgb[dst * 3 + 2] = v.z;
+  }
+}
+
+__global__ void pack_rgb_kernel(const float4* __restrict__ in, float* __restrict__ out, long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float4 v = in[i];
+  out[i * 3] = v.x;
+  out[i * 3 + 1] = v.y;
+  out[i * 3 + 2] = v.z;
+}
+
+__global__ void unpack_rgb_kernel(const float* __restrict__ in, float4* __restrict__ out, long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  out[i] = make_float4(in[i * 3], in[i * 3 + 1], in[i * 3 + 2], 0.f);
+}
+
+inline unsigned blocks_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t launch_transmittance(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream) {
+  long long n = (long long)P.nT[0] * P.nT[1];
+  hipLaunchKernelGGL(transmittance_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, out, wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream) {
+  long long n = (long long)P.nE[0] * P.nE[1];
+  hipLaunchKernelGGL(surface_radiance_base_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, out, wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
+                                 hipStream_t stream) {
+  unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
+  hipLaunchKernelGGL(single_scatter_kernel, dim3(rays), dim3(kBlock), 0, stream, P, rayleigh, mie, wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
+                                const float4* dE, float4* out, WorkCounters* wc, hipStream_t stream) {
+  unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
+  size_t shmem = sizeof(DirData) * (size_t)ndirs;
+  hipLaunchKernelGGL(point_scatter_kernel, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, ndirs, dS, mie, dE, out,
+                     wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
+                              hipStream_t stream) {
+  unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
+  hipLaunchKernelGGL(ray_scatter_kernel, dim3(rays), dim3(kBlock), 0, stream, P, dJ, out, wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
+                                   float4* out, WorkCounters* wc, hipStream_t stream) {
+  unsigned entries = (unsigned)(P.nE[0] * P.nE[1]);
+  hipLaunchKernelGGL(surface_radiance_kernel, dim3(entries), dim3(kBlock), 0, stream, P, dirs, ndirs, dS, mie, out,
+                     wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
+                           float* out_rgb, int tiled, hipStream_t stream) {
+  long long n = space == 4   ? (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3]
+                : space == 2 ? (long long)P.nT[0] * P.nT[1]
+                             : (long long)P.nE[0] * P.nE[1];
+  hipLaunchKernelGGL(resample_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, space, a, b, out4, out_rgb,
+                     tiled);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_rgb(const float4* in, float* out, long long count, hipStream_t stream) {
+  hipLaunchKernelGGL(pack_rgb_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, in, out, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_rgb(const float* in, float4* out, long long count, hipStream_t stream) {
+  hipLaunchKernelGGL(unpack_rgb_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, in, out, count);
+  return hipGetLastError();
+}
+
+}  // namespace sfsim

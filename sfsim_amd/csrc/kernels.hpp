@@ -1,0 +1,32 @@
+// Launchers of the table kernels (kernels.hip).  All take device pointers and enqueue on `stream`.
+// Tables are float4 per entry (RGB + pad) in grid order unless stated otherwise.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_params.hpp"
+
+namespace sfsim {
+
+hipError_t launch_transmittance(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream);
+hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream);
+hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
+                                 hipStream_t stream);
+// dS: previous-order scatter table; mie (nullable): first-iteration Mie strength table, added with the
+// Mie phase of the quadrature direction; dE: previous-order surface-radiance table.
+hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
+                                const float4* dE, float4* out, WorkCounters* wc, hipStream_t stream);
+hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
+                              hipStream_t stream);
+hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
+                                   float4* out, WorkCounters* wc, hipStream_t stream);
+// Re-sample a (+ b when non-null) at forward(backward(i)) of the space (2, 3 or 4).  Output either as
+// float4 grid-order table (out4) or as packed RGB floats (out_rgb): grid order, or for space 4 with
+// tiled != 0 the 4D->2D tiling of the .scatter files.
+hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
+                           float* out_rgb, int tiled, hipStream_t stream);
+// Pack a float4 grid-order table into RGB floats (grid order), for the stage entry points.
+hipError_t launch_pack_rgb(const float4* in, float* out, long long count, hipStream_t stream);
+hipError_t launch_unpack_rgb(const float* in, float4* out, long long count, hipStream_t stream);
+
+}  // namespace sfsim

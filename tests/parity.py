@@ -1,0 +1,44 @@
+"""Shared helpers of the parity tests: error metrics and oracle parameter conversion.
+
+Relative error of a table against the FP64 oracle: rel = |gpu - oracle| / max(|oracle|, floor) with
+floor = FLOOR_FRACTION * max|oracle| taken per colour channel over the table, so entries that are
+zero in the reference (sun below the horizon, night side) are compared against the table's scale
+instead of dividing by zero.
+"""
+import numpy as np
+
+FLOOR_FRACTION = 1e-6
+TOL_TRANSMITTANCE = 1e-4
+TOL_SCATTER = 1e-3
+
+
+def rel_errors(gpu, ref, floor_fraction=FLOOR_FRACTION, channel_max=None):
+    """channel_max: per-channel maximum of the whole reference table when ``ref`` is a subset."""
+    gpu = np.asarray(gpu, np.float64).reshape(-1, 3)
+    ref = np.asarray(ref, np.float64).reshape(-1, 3)
+    cmax = np.max(np.abs(ref), axis=0) if channel_max is None else np.asarray(channel_max, np.float64)
+    floor = floor_fraction * cmax
+    floor = np.where(floor > 0, floor, np.finfo(np.float64).tiny)
+    return np.abs(gpu - ref) / np.maximum(np.abs(ref), floor)
+
+
+def report(name, gpu, ref, channel_max=None):
+    e = rel_errors(gpu, ref, channel_max=channel_max)
+    return {"table": name, "max_rel": float(e.max()), "rms_rel": float(np.sqrt(np.mean(e * e)))}
+
+
+def assert_close(name, gpu, ref, tol, channel_max=None):
+    r = report(name, gpu, ref, channel_max)
+    assert np.all(np.isfinite(np.asarray(gpu))), f"{name}: non-finite values"
+    assert r["max_rel"] <= tol, f"{name}: max rel err {r['max_rel']:.3e} > {tol:.0e} (rms {r['rms_rel']:.3e})"
+    return r
+
+
+def oracle_params(oracle, P):
+    """Copy a sfsim_amd.atmosphere.Params into the oracle's parameter struct."""
+    planet = oracle.planet(P.planet.radius, P.planet.height, tuple(P.planet.brightness), tuple(P.planet.centre))
+    mie = oracle.scatter(tuple(P.mie.scatter_base), P.mie.scatter_scale, P.mie.scatter_quotient, P.mie.scatter_g)
+    ray = oracle.scatter(tuple(P.rayleigh.scatter_base), P.rayleigh.scatter_scale, P.rayleigh.scatter_quotient,
+                         P.rayleigh.scatter_g)
+    return oracle.params(planet, mie, ray, tuple(P.transmittance_shape), tuple(P.surface_radiance_shape),
+                         tuple(P.scatter_shape), P.ray_steps, P.sphere_steps, P.iterations, tuple(P.intensity))

@@ -1,0 +1,296 @@
+"""GPU parity of libsfsim_atmo against the FP64 oracle (and, through it, the reference).
+
+* reference table probes: the size-12 transmittance and first-order scatter tables and the analytic
+  surface-radiance table of the reference's tests, built on the GPU and read through the oracle's
+  restatement of the renderer's shader functions, against the reference's own expected values;
+* every stage at small sizes, fed identical inputs, against the oracle's stage (whole tables);
+* the whole driver (3 orders) at small sizes against the oracle's driver;
+* every stage at the full 5-order workload (T 64x256, E 16x64, S 32x64x32x8, 64 steps), checked on
+  seeded random entry subsets with the GPU's own previous-stage tables as inputs;
+* error paths that the host must turn into error codes.
+
+Tolerances: transmittance 1e-4, everything else 1e-3 relative (floor: 1e-6 of the table's channel
+maximum, see tests/parity.py).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.parity import TOL_SCATTER, TOL_TRANSMITTANCE, assert_close, oracle_params, rel_errors
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def atm():
+    from sfsim_amd import atmosphere
+    return atmosphere
+
+
+@pytest.fixture(scope="module")
+def gpu(atm):
+    ctx = atm.Atmosphere(0)
+    yield ctx
+    ctx.close()
+
+
+def small_params(atm, iterations=3, height=100000.0, brightness=0.3):
+    return atm.make_params(planet={"height": height, "brightness": (brightness,) * 3}, transmittance_shape=(10, 17),
+                           surface_radiance_shape=(6, 11), scatter_shape=(6, 11, 7, 4), ray_steps=16, sphere_steps=10,
+                           iterations=iterations)
+
+
+# ------------------------------------------------------------------ reference table probes
+def test_reference_size12_table_probes(atm, gpu, oracle):
+    """t_atmosphere.clj:569-578, 629-635, 661-667, 769-775 with the tables built on the GPU."""
+    golden = json.load(open(os.path.join(HERE, "golden", "reference_goldens.json")))
+    P = atm.make_params(planet={"height": 100000.0}, transmittance_shape=(12, 12), surface_radiance_shape=(12, 12),
+                        scatter_shape=(12, 12, 12, 12), ray_steps=10, sphere_steps=8, iterations=0)
+    T = gpu.transmittance_table(P).astype(np.float64).reshape(-1, 3)
+    R1, _ = gpu.single_scatter_tables(P)
+    S = R1.astype(np.float64).reshape(-1, 3)
+    earth = oracle.planet(6378000.0, 100000.0)
+    checked = 0
+    for case in golden["cases"]:
+        if case["fn"] == "table_transmittance_track":
+            got = oracle.glsl_transmittance_track(earth, (12, 12), T, case["p"], case["q"])
+        elif case["fn"] == "table_transmittance_outer":
+            got = oracle.glsl_transmittance_outer(earth, (12, 12), T, case["p"], case["d"])
+        elif case["fn"] == "table_ray_scatter_track":
+            got = oracle.glsl_ray_scatter_track(earth, (12, 12, 12, 12), S, S, (12, 12), T, 0.76, case["light"],
+                                                case["p"], case["q"])
+        else:
+            continue
+        assert abs(got[case["channel"]] - case["expected"]) <= case["tol"], case["src"]
+        checked += 1
+    assert checked == 9
+
+
+# ------------------------------------------------------------------ stages at small sizes
+def test_stage_transmittance_small(atm, gpu, oracle):
+    P = small_params(atm)
+    assert_close("T", gpu.transmittance_table(P), oracle.stage_transmittance(oracle_params(oracle, P)),
+                 TOL_TRANSMITTANCE)
+
+
+def test_stage_surface_radiance_base_small(atm, gpu, oracle):
+    P = small_params(atm)
+    assert_close("E0", gpu.surface_radiance_base_table(P),
+                 oracle.stage_surface_radiance_base(oracle_params(oracle, P)), TOL_TRANSMITTANCE)
+
+
+def test_stage_single_scatter_small(atm, gpu, oracle):
+    P = small_params(atm)
+    OP = oracle_params(oracle, P)
+    R1, M1 = gpu.single_scatter_tables(P)
+    assert_close("R1", R1, oracle.stage_single_scatter(OP, 0), TOL_SCATTER)
+    assert_close("M1", M1, oracle.stage_single_scatter(OP, 1), TOL_SCATTER)
+
+
+@pytest.fixture(scope="module")
+def small_inputs(atm, oracle):
+    P = small_params(atm)
+    OP = oracle_params(oracle, P)
+    R1 = oracle.stage_single_scatter(OP, 0)
+    M1 = oracle.stage_single_scatter(OP, 1)
+    E0 = oracle.stage_surface_radiance_base(OP)
+    J1 = oracle.stage_point_scatter(OP, 1, R1, M1, E0)
+    return P, OP, R1, M1, E0, J1
+
+
+def test_stage_point_scatter_small(gpu, oracle, small_inputs):
+    P, OP, R1, M1, E0, J1 = small_inputs
+    # first iteration: R1 + M1 * phase(mie)
+    assert_close("dJ1", gpu.point_scatter_table(P, R1, E0, mie_strength=M1), J1, TOL_SCATTER)
+    # later iterations: a plain dS table
+    dS = oracle.stage_ray_scatter(OP, J1)
+    assert_close("dJ2", gpu.point_scatter_table(P, dS, E0), oracle.stage_point_scatter(OP, 0, dS, None, E0),
+                 TOL_SCATTER)
+
+
+def test_stage_ray_scatter_small(gpu, oracle, small_inputs):
+    P, OP, R1, M1, E0, J1 = small_inputs
+    assert_close("dS", gpu.ray_scatter_table(P, J1), oracle.stage_ray_scatter(OP, J1), TOL_SCATTER)
+
+
+def test_stage_surface_radiance_small(gpu, oracle, small_inputs):
+    P, OP, R1, M1, E0, J1 = small_inputs
+    assert_close("dE1", gpu.surface_radiance_table(P, R1, mie_strength=M1),
+                 oracle.stage_surface_radiance(OP, 1, R1, M1), TOL_SCATTER)
+    dS = oracle.stage_ray_scatter(OP, J1)
+    assert_close("dE2", gpu.surface_radiance_table(P, dS), oracle.stage_surface_radiance(OP, 0, dS, None),
+                 TOL_SCATTER)
+
+
+@pytest.mark.parametrize("space", [2, 3, 4])
+def test_stage_resample_small(atm, gpu, oracle, space):
+    P = small_params(atm)
+    OP = oracle_params(oracle, P)
+    shape = {2: P.transmittance_shape, 3: P.surface_radiance_shape, 4: P.scatter_shape}[space]
+    rng = np.random.default_rng(20261202 + space)
+    a = rng.uniform(0.1, 1.0, (int(np.prod(shape)), 3))
+    b = rng.uniform(0.1, 1.0, (int(np.prod(shape)), 3))
+    a32, b32 = a.astype(np.float32), b.astype(np.float32)
+    assert_close(f"resample{space}", gpu.resample_table(P, space, a32, b32),
+                 oracle.stage_resample(OP, space, a32.astype(np.float64), b32.astype(np.float64)), 1e-6)
+
+
+# ------------------------------------------------------------------ whole driver, small
+@pytest.mark.parametrize("iterations,brightness", [(0, 0.3), (1, 0.3), (3, 0.3), (2, 0.0)])
+def test_generate_luts_small(atm, gpu, oracle, iterations, brightness):
+    P = small_params(atm, iterations=iterations, brightness=brightness)
+    OP = oracle_params(oracle, P)
+    T, E, S, M = oracle.generate_atmosphere_luts(OP)
+    got = gpu.generate_atmosphere_luts(P)
+    assert_close("transmittance", got["transmittance"], T, TOL_TRANSMITTANCE)
+    if iterations:
+        assert_close("surface-radiance", got["surface-radiance"], E, TOL_SCATTER)
+    else:
+        assert not got["surface-radiance"].any()
+    assert_close("ray-scatter", got["ray-scatter"], oracle.convert_4d_to_2d(S), TOL_SCATTER)
+    assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
+
+
+def test_build_fetch_matches_generate(atm, gpu):
+    P = small_params(atm, iterations=2)
+    ref = gpu.generate_atmosphere_luts(P)
+    gpu.build(P)
+    for which, name in enumerate(["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]):
+        assert np.array_equal(gpu.fetch(P, which), ref[name]), name
+    timings = gpu.stage_timings()
+    assert timings["single_scatter"][1] == 1 and timings["point_scatter"][1] == 2
+
+
+def test_determinism(atm, gpu):
+    P = small_params(atm, iterations=2)
+    a = gpu.generate_atmosphere_luts(P)
+    b = gpu.generate_atmosphere_luts(P)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+# ------------------------------------------------------------------ full-size stages on subsets
+@pytest.fixture(scope="module")
+def full(atm, gpu):
+    P = atm.workload("full-5-order")
+    T = gpu.transmittance_table(P)
+    E0 = gpu.surface_radiance_base_table(P)
+    R1, M1 = gpu.single_scatter_tables(P)
+    J1 = gpu.point_scatter_table(P, R1, E0, mie_strength=M1)
+    dE1 = gpu.surface_radiance_table(P, R1, mie_strength=M1)
+    dS1 = gpu.ray_scatter_table(P, J1)
+    J2 = gpu.point_scatter_table(P, dS1, dE1)
+    return P, T, E0, R1, M1, J1, dE1, dS1, J2
+
+
+def _subset(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, size=min(k, n), replace=False))
+
+
+def test_full_transmittance(oracle, full):
+    P, T = full[0], full[1]
+    assert_close("T", T, oracle.stage_transmittance(oracle_params(oracle, P)), TOL_TRANSMITTANCE)
+
+
+def test_full_surface_radiance_base(oracle, full):
+    P, E0 = full[0], full[2]
+    assert_close("E0", E0, oracle.stage_surface_radiance_base(oracle_params(oracle, P)), TOL_TRANSMITTANCE)
+
+
+def test_full_single_scatter_subset(oracle, full):
+    P, R1, M1 = full[0], full[3], full[4]
+    OP = oracle_params(oracle, P)
+    idx = _subset(R1.size // 3, 384, 1)
+    assert_close("R1", R1.reshape(-1, 3)[idx], oracle.stage_single_scatter(OP, 0, idx), TOL_SCATTER)
+    assert_close("M1", M1.reshape(-1, 3)[idx], oracle.stage_single_scatter(OP, 1, idx), TOL_SCATTER)
+
+
+def test_full_point_scatter_subset(oracle, full):
+    P, T, E0, R1, M1, J1, dE1, dS1, J2 = full
+    OP = oracle_params(oracle, P)
+    idx = _subset(R1.size // 3, 384, 2)
+    f64 = lambda a: a.astype(np.float64).reshape(-1, 3)
+    ref1 = oracle.stage_point_scatter(OP, 1, f64(R1), f64(M1), f64(E0), idx)
+    assert_close("dJ1", J1.reshape(-1, 3)[idx], ref1, TOL_SCATTER)
+    ref2 = oracle.stage_point_scatter(OP, 0, f64(dS1), None, f64(dE1), idx)
+    assert_close("dJ2", J2.reshape(-1, 3)[idx], ref2, TOL_SCATTER)
+
+
+def test_full_ray_scatter_subset(oracle, full):
+    P, J1, dS1 = full[0], full[5], full[7]
+    OP = oracle_params(oracle, P)
+    idx = _subset(J1.size // 3, 384, 3)
+    assert_close("dS1", dS1.reshape(-1, 3)[idx], oracle.stage_ray_scatter(OP, J1.astype(np.float64).reshape(-1, 3), idx),
+                 TOL_SCATTER)
+
+
+def test_full_surface_radiance(oracle, full):
+    P, R1, M1, dE1 = full[0], full[3], full[4], full[6]
+    OP = oracle_params(oracle, P)
+    ref = oracle.stage_surface_radiance(OP, 1, R1.astype(np.float64).reshape(-1, 3), M1.astype(np.float64).reshape(-1, 3))
+    assert_close("dE1", dE1, ref, TOL_SCATTER)
+
+
+def test_full_build_is_finite_and_positive(atm, gpu):
+    P = atm.workload("full-5-order")
+    luts = gpu.generate_atmosphere_luts(P)
+    for name, t in luts.items():
+        assert np.all(np.isfinite(t)), name
+        assert np.all(t >= 0), name
+    assert np.all(luts["transmittance"] <= 1.0)
+
+
+# ------------------------------------------------------------------ error paths
+def test_invalid_params_are_errors(atm, gpu):
+    P = small_params(atm)
+    P.scatter_shape[2] = 1
+    with pytest.raises(atm.AtmoError) as e:
+        gpu.transmittance_table(P)
+    assert e.value.code == atm.ERR_ARGUMENT
+    P = small_params(atm)
+    P.ray_steps = 0
+    with pytest.raises(atm.AtmoError) as e:
+        gpu.build(P)
+    assert e.value.code == atm.ERR_ARGUMENT
+    P = small_params(atm)
+    P.mie.scatter_scale = float("nan")
+    with pytest.raises(atm.AtmoError) as e:
+        gpu.build(P)
+    assert e.value.code == atm.ERR_NONFINITE
+
+
+def test_nonfinite_input_table_is_an_error(atm, gpu):
+    P = small_params(atm)
+    n = int(np.prod(P.scatter_shape))
+    dJ = np.ones((n, 3), np.float32)
+    dJ[5, 1] = np.nan
+    with pytest.raises(atm.AtmoError) as e:
+        gpu.ray_scatter_table(P, dJ)
+    assert e.value.code == atm.ERR_NONFINITE
+
+
+def test_fetch_before_build_is_an_error(atm):
+    ctx = atm.Atmosphere(0)
+    P = small_params(atm)
+    with pytest.raises(atm.AtmoError):
+        ctx.fetch(P, 0)
+    ctx.close()
+
+
+# ------------------------------------------------------------------ full-size end to end
+def test_full_build_against_oracle_fixture(atm, gpu):
+    """Seeded subset of the FP64 oracle's full 5-order build (tests/golden/make_oracle_subsets.py)."""
+    path = os.path.join(HERE, "golden", "oracle_full-5-order.npz")
+    if not os.path.exists(path):
+        pytest.skip("oracle fixture not generated")
+    ref = np.load(path)
+    luts = gpu.generate_atmosphere_luts(atm.workload("full-5-order"))
+    for name, tol in (("transmittance", TOL_TRANSMITTANCE), ("surface-radiance", TOL_SCATTER),
+                      ("ray-scatter", TOL_SCATTER), ("mie-strength", TOL_SCATTER)):
+        idx = ref[name + "/index"]
+        assert_close(name, luts[name].reshape(-1, 3)[idx], ref[name + "/value"], tol,
+                     channel_max=ref[name + "/channel_max"])
