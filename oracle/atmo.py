@@ -71,6 +71,15 @@ def lib():
         _lib.orc_scattering.argtypes = [ctypes.POINTER(Scatter), ctypes.c_double, _dp]
         _lib.orc_extinction.argtypes = [ctypes.POINTER(Scatter), ctypes.c_double, _dp]
         _lib.orc_max_threads.restype = ctypes.c_int
+        PP, i32, i64 = ctypes.POINTER(Params), ctypes.c_int, ctypes.c_int64
+        _lib.orc_stage_transmittance.argtypes = [PP, _lp, i64, _dp, i32, i32]
+        _lib.orc_stage_surface_radiance_base.argtypes = [PP, _lp, i64, _dp, i32, i32]
+        _lib.orc_stage_single_scatter.argtypes = [PP, i32, _lp, i64, _dp, i32, i32]
+        _lib.orc_stage_point_scatter.argtypes = [PP, i32, _dp, _dp, _dp, _lp, i64, _dp, i32, i32]
+        _lib.orc_stage_ray_scatter.argtypes = [PP, _dp, _lp, i64, _dp, i32, i32]
+        _lib.orc_stage_surface_radiance.argtypes = [PP, i32, _dp, _dp, _lp, i64, _dp, i32, i32]
+        _lib.orc_stage_resample.argtypes = [PP, i32, _dp, _dp, _lp, i64, _dp, i32]
+        _lib.orc_generate_atmosphere_luts.argtypes = [PP, _dp, _dp, _dp, _dp, i32, i32]
     return _lib
 
 

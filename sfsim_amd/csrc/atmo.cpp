@@ -179,6 +179,7 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   P.inv_scale_ray = 1.0 / p->rayleigh.scatter_scale;
   P.g_mie = (float)p->mie.scatter_g;
   P.g_ray = (float)p->rayleigh.scatter_g;
+  P.sun_inv_den = (float)(1.0 / (1.0 - std::exp(-3.6)));
   for (int d = 0; d < 2; ++d) {
     P.nT[d] = p->transmittance_shape[d];
     P.nE[d] = p->surface_radiance_shape[d];

@@ -17,6 +17,7 @@ struct DevParams {
   float k_mie, k_ray;
   double inv_scale_mie, inv_scale_ray;
   float g_mie, g_ray;
+  float sun_inv_den;  // 1 / (1 - exp(-3.6)) of sun-elevation-to-index (atmosphere.clj:322-326)
   float brightness_over_pi[3];
   float intensity[3];
   int nT[2];  // transmittance shape  [height elevation]

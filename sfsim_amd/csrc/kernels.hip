@@ -173,10 +173,9 @@ __device__ __forceinline__ float3 lookup4(const float4* __restrict__ T, int N1, 
   return mix3(r[0], r[1], a.s);
 }
 
-__device__ __forceinline__ float sun_elevation_coord(int size, float sin_elevation) {
-  // atmosphere.clj:322-326; 1 / (1 - exp(-3.6))
-  const float inv_den = 1.0280583777619436f;
-  return (size - 1) * fmaxf(0.0f, (1.0f - __expf(-3.0f * sin_elevation - 0.6f)) * inv_den);
+// atmosphere.clj:322-326 in FP32
+__device__ __forceinline__ float sun_elevation_coord(const DevParams& P, int size, float sin_elevation) {
+  return (size - 1) * fmaxf(0.0f, (1.0f - __expf(-3.0f * sin_elevation - 0.6f)) * P.sun_inv_den);
 }
 
 __device__ __forceinline__ void count(WorkCounters* wc, unsigned long long single, unsigned long long samples,
@@ -335,7 +334,7 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
       float2 gi = s_g[i];
       float sin_e = fmaf(gi.y, vl, xl) * s_rinv[i];
       float3 v = lookup4(dJ, NE, NL, NA, make_lerp(w.w, NH), make_lerp(gi.x, NE),
-                         make_lerp(sun_elevation_coord(NL, sin_e), NL), La);
+                         make_lerp(sun_elevation_coord(P, NL, sin_e), NL), La);
       acc.x += w.x * v.x; acc.y += w.y * v.y; acc.z += w.z * v.z;
     }
     lookups += n;
@@ -424,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
       }
       if (dd.ground.w != 0.f) {
         float spl = dd.phat.x * lx + dd.phat.y * ly + dd.phat.z * lz;
-        float3 e = lookup2(dE, P.nE[1], make_lerp(dd.phat.w, P.nE[0]), make_lerp(sun_elevation_coord(P.nE[1], spl), P.nE[1]));
+        float3 e = lookup2(dE, P.nE[1], make_lerp(dd.phat.w, P.nE[0]), make_lerp(sun_elevation_coord(P, P.nE[1], spl), P.nE[1]));
         v.x += dd.ground.x * e.x; v.y += dd.ground.y * e.y; v.z += dd.ground.z * e.z;
         ++l2;
       }

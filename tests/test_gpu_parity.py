@@ -136,7 +136,7 @@ def test_stage_resample_small(atm, gpu, oracle, space):
     b = rng.uniform(0.1, 1.0, (int(np.prod(shape)), 3))
     a32, b32 = a.astype(np.float32), b.astype(np.float32)
     assert_close(f"resample{space}", gpu.resample_table(P, space, a32, b32),
-                 oracle.stage_resample(OP, space, a32.astype(np.float64), b32.astype(np.float64)), 1e-6)
+                 oracle.stage_resample(OP, space, a32.astype(np.float64), b32.astype(np.float64)), 1e-5)
 
 
 # ------------------------------------------------------------------ whole driver, small
