@@ -125,6 +125,9 @@ int validate(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   if (pl.centre[0] != 0.0 || pl.centre[1] != 0.0 || pl.centre[2] != 0.0)
     return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "planet centre must be the origin (atmosphere.clj:95-102 assumes it)");
   if (!(pl.radius > 0.0) || !(pl.height > 0.0)) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "radius and height must be > 0");
+  // the FP64 altitude series needs (Rt^2 - R^2) / R^2 <= 0.1
+  if (pl.height * (2.0 * pl.radius + pl.height) > 0.1 * pl.radius * pl.radius)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "atmosphere height must be below 4.9% of the radius");
   for (const sfsim_atmo_scatter* c : comps) {
     if (!(c->scatter_scale > 0.0)) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "scatter-scale must be > 0");
     if (!(c->scatter_quotient > 0.0)) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "scatter-quotient must be > 0");
@@ -187,6 +190,7 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   for (int d = 0; d < 4; ++d) P.nS[d] = p->scatter_shape[d];
   P.ray_steps = p->ray_steps;
   P.graze_height = 2000.0;
+  P.graze_u = P.graze_height * (2.0 * P.g.R + P.graze_height);
   return P;
 }
 

@@ -24,9 +24,10 @@ struct DevParams {
   int nE[2];  // surface radiance shape [height sun-elevation]
   int nS[4];  // scatter shape [height elevation sun-elevation heading]
   int ray_steps;
-  // optical depth of a segment whose lowest point lies strictly inside it and below this altitude
-  // is accumulated in FP64 (near-horizon / grazing rays); everything else in FP32 + compensation
-  double graze_height;
+  // optical depth of a segment whose lowest point lies strictly inside it and below the altitude
+  // graze_height (u = r^2 - R^2 < graze_u there) is accumulated in FP64 (near-horizon / grazing
+  // rays); everything else in FP32 with compensated sums
+  double graze_height, graze_u;
 };
 
 // Quadrature direction (FP64 unit vector in the frame of a grid point on +x) and its weight.

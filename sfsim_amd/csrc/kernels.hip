@@ -37,13 +37,37 @@ __device__ __forceinline__ float phase_f(float g, float mu) {
   return (3.0f * (1.0f - g2) * (1.0f + mu * mu)) / (8.0f * 3.14159265358979f * (2.0f + g2) * (x * sqrtf(x)));
 }
 
-// Sums over midpoint nodes t_j = (j + 1/2) dt, j < n, of exp(-h_j / H_mie) and exp(-h_j / H_rayleigh)
-// along a ray from altitude h0 (c0 = h0 (2R + h0)) with b = 2 dot(point, unit direction):
-// h(t) = (c0 + t (b + t)) / (R + sqrt(R^2 + c0 + t (b + t))).  8-node partial sums, Kahan across them.
-__device__ float2 density_sums_f32(const DevParams& P, float c0, float b, float dt, int n) {
+// Altitude along a ray, centred on the segment's lowest point.  With u(t) = r(t)^2 - R^2 =
+// h0 (2R + h0) + t (2 pd0 + t) and any shift t = s + ts:  u = u(ts) + s (b' + s), b' = 2 pd0 + 2 ts.
+// Taking ts = clamp(-pd0, 0, D) (the point of the segment closest to the planet centre) leaves no
+// cancellation: near ts, where the density is largest, u ~ u(ts) is exact, and elsewhere s and
+// b' + s have the same sign.  h = u / (R + sqrt(R^2 + u)).
+struct Segment {
+  float c0, b, s0, dt;  // FP32 node constants: s_j = j dt + s0, u_j = c0 + s_j (b + s_j)
+  double c0d, bd, s0d, dtd;
+  bool graze;           // lowest point interior and below the FP64 altitude threshold
+};
+
+__device__ __forceinline__ Segment make_segment(const DevParams& P, double h0, double pd0, double D, int n) {
+  Segment g;
+  double ts = fmin(fmax(-pd0, 0.0), D);
+  g.c0d = h0 * (2.0 * P.g.R + h0) + ts * (2.0 * pd0 + ts);
+  g.bd = 2.0 * pd0 + 2.0 * ts;
+  g.dtd = D / n;
+  g.s0d = 0.5 * g.dtd - ts;
+  g.c0 = (float)g.c0d;
+  g.b = (float)g.bd;
+  g.s0 = (float)g.s0d;
+  g.dt = (float)g.dtd;
+  g.graze = (-pd0 > 0.0) && (-pd0 < D) && (g.c0d < P.graze_u);
+  return g;
+}
+
+// Sums over the n midpoint nodes of exp(-h / H_mie) and exp(-h / H_rayleigh) with FP32 altitudes;
+// 8-node partial sums, Kahan-compensated across them.
+__device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) {
   const float R = (float)P.g.R;
   const float R2 = (float)(P.g.R * P.g.R);
-  const float half = 0.5f * dt;
   const float km = P.k_mie, kr = P.k_ray;
   float sm = 0.f, sr = 0.f, cm = 0.f, cr = 0.f;
   int j = 0;
@@ -51,8 +75,8 @@ __device__ float2 density_sums_f32(const DevParams& P, float c0, float b, float 
     float pm = 0.f, pr = 0.f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      float t = fmaf((float)(j + q), dt, half);
-      float u = fmaf(t, t + b, c0);
+      float s = fmaf((float)(j + q), g.dt, g.s0);
+      float u = fmaf(s, s + g.b, g.c0);
       float h = u * frcp(R + fsqrt(R2 + u));
       pm += fexp2(h * km);
       pr += fexp2(h * kr);
@@ -63,8 +87,8 @@ __device__ float2 density_sums_f32(const DevParams& P, float c0, float b, float 
   if (j < n) {
     float pm = 0.f, pr = 0.f;
     for (; j < n; ++j) {
-      float t = fmaf((float)j, dt, half);
-      float u = fmaf(t, t + b, c0);
+      float s = fmaf((float)j, g.dt, g.s0);
+      float u = fmaf(s, s + g.b, g.c0);
       float h = u * frcp(R + fsqrt(R2 + u));
       pm += fexp2(h * km);
       pr += fexp2(h * kr);
@@ -75,45 +99,53 @@ __device__ float2 density_sums_f32(const DevParams& P, float c0, float b, float 
   return make_float2(sm, sr);
 }
 
-__device__ double2 density_sums_f64(const DevParams& P, double h0, double pd0, double dt, int n) {
-  const double R = P.g.R;
-  const double c0 = h0 * (2.0 * R + h0), b = 2.0 * pd0;
+// Same sums with FP64 node positions, altitudes and accumulation (near-horizon and grazing
+// segments).  h = (u / 2R) * 2 (sqrt(1 + x) - 1) / x, x = u / R^2, by its Taylor series
+// (|x| <= (Rt^2 - R^2) / R^2 <= 0.1 is checked on the host; 10 terms leave < 2e-12 relative);
+// the exponentials are FP32 (their argument carries the altitude's FP32 rounding, 6e-8).
+__device__ double2 density_sums_f64(const DevParams& P, const Segment& g, int n) {
+  const double inv_R2 = 1.0 / (P.g.R * P.g.R), inv_2R = 0.5 / P.g.R;
+  const float km = P.k_mie, kr = P.k_ray;
   double sm = 0.0, sr = 0.0;
   for (int j = 0; j < n; ++j) {
-    double t = (j + 0.5) * dt;
-    double u = c0 + t * (b + t);
-    double h = u / (R + sqrt(R * R + u));
-    sm += exp(-h * P.inv_scale_mie);
-    sr += exp(-h * P.inv_scale_ray);
+    double s = fma((double)j, g.dtd, g.s0d);
+    double u = fma(s, s + g.bd, g.c0d);
+    double x = u * inv_R2;
+    // 2 (sqrt(1 + x) - 1) / x = sum_k 2 binom(1/2, k + 1) x^k, k = 0..9
+    double p = -0.01854705810546875;
+    p = fma(p, x, 0.021820068359375);
+    p = fma(p, x, -0.02618408203125);
+    p = fma(p, x, 0.0322265625);
+    p = fma(p, x, -0.041015625);
+    p = fma(p, x, 0.0546875);
+    p = fma(p, x, -0.078125);
+    p = fma(p, x, 0.125);
+    p = fma(p, x, -0.25);
+    p = fma(p, x, 1.0);
+    float h = (float)(u * inv_2R * p);
+    sm += (double)fexp2(h * km);
+    sr += (double)fexp2(h * kr);
   }
   return make_double2(sm, sr);
 }
 
-// True when the lowest point of the segment [0, D] lies strictly inside it and below graze_height.
-__device__ __forceinline__ bool grazing(const DevParams& P, double h0, double pd0, double D) {
-  double tp = -pd0;
-  if (!(tp > 0.0 && tp < D)) return false;
-  double u = h0 * (2.0 * P.g.R + h0) - pd0 * pd0;
-  double hp = u / (P.g.R + sqrt(P.g.R * P.g.R + u));
-  return hp < P.graze_height;
-}
-
 // Transmittance of the segment of length D from altitude h0 along a unit direction with
-// dot(point, direction) = pd0, n midpoint samples (atmosphere.clj:114-125).
+// dot(point, direction) = pd0, n midpoint samples (atmosphere.clj:114-125).  The FP64 variant is
+// chosen per wavefront (when any active lane holds a grazing segment) so that the two loops never
+// diverge within a wave.
 __device__ float3 segment_transmittance(const DevParams& P, double h0, double pd0, double D, int n) {
-  if (!(D > 0.0)) return make_float3(1.f, 1.f, 1.f);
-  double dtd = D / n;
+  Segment g = make_segment(P, h0, pd0, D, n);
   float sm, sr;
-  if (grazing(P, h0, pd0, D)) {
-    double2 s = density_sums_f64(P, h0, pd0, dtd, n);
-    sm = (float)(s.x * dtd);
-    sr = (float)(s.y * dtd);
+  if (__any(g.graze)) {
+    double2 s = density_sums_f64(P, g, n);
+    sm = (float)(s.x * g.dtd);
+    sr = (float)(s.y * g.dtd);
   } else {
-    float dt = (float)dtd;
-    float2 s = density_sums_f32(P, (float)(h0 * (2.0 * P.g.R + h0)), (float)(2.0 * pd0), dt, n);
-    sm = s.x * dt;
-    sr = s.y * dt;
+    float2 s = density_sums_f32(P, g, n);
+    sm = s.x * g.dt;
+    sr = s.y * g.dt;
   }
+  if (!(D > 0.0)) return make_float3(1.f, 1.f, 1.f);
   return make_float3(expf(-(P.ext_mie[0] * sm + P.ext_ray[0] * sr)), expf(-(P.ext_mie[1] * sm + P.ext_ray[1] * sr)),
                      expf(-(P.ext_mie[2] * sm + P.ext_ray[2] * sr)));
 }
