@@ -21,6 +21,7 @@ enum Kind {
   kTransmittance = 0,
   kSurfaceRadianceBase,
   kSingleScatter,
+  kDirectionGeometry,
   kPointScatter,
   kSurfaceRadiance,
   kRayScatter,
@@ -29,9 +30,9 @@ enum Kind {
   kNumKinds
 };
 
-const char* const kKindNames[kNumKinds] = {"transmittance", "surface_radiance_base", "single_scatter",
-                                           "point_scatter", "surface_radiance",      "ray_scatter",
-                                           "accumulate",    "final_resample"};
+const char* const kKindNames[kNumKinds] = {"transmittance",      "surface_radiance_base", "single_scatter",
+                                           "direction_geometry", "point_scatter",         "surface_radiance",
+                                           "ray_scatter",        "accumulate",            "final_resample"};
 
 struct DeviceBuffer {
   void* ptr = nullptr;
@@ -54,6 +55,7 @@ struct sfsim_atmo_ctx {
   DeviceBuffer stage_in[3], stage_out4, stage_rgb;
   // quadrature directions
   DeviceBuffer sphere_dirs, half_dirs;
+  DeviceBuffer geom, slab, slab_mie;  // point-scatter direction geometry and folded dS slabs
   int sphere_key = -1, half_key = -1, sphere_n = 0, half_n = 0;
   DeviceBuffer counters;
   // timing
@@ -142,6 +144,8 @@ int validate(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   if (count4(p->scatter_shape) > (1LL << 31) / 4 || count2(p->transmittance_shape) > (1LL << 30) ||
       count2(p->surface_radiance_shape) > (1LL << 30))
     return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "table too large");
+  if ((long long)p->scatter_shape[2] * p->scatter_shape[3] > 1024)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sun-elevation x heading size must be <= 1024");
   if (p->ray_steps < 1 || p->ray_steps > kMaxSteps)
     return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "ray_steps must be in [1, 256]");
   if (p->iterations < 0 || p->iterations > 1000) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "iterations must be in [0, 1000]");
@@ -152,7 +156,7 @@ int validate(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
 // (sphere.clj:96-105); with none the reference's reduction over no samples fails.
 int validate_sphere(sfsim_atmo_ctx* ctx, int steps) {
   if ((steps >> 1) < 1 || sfsim_atmo_sphere_directions(0, steps) > kMaxSphereDirs)
-    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sphere_steps must give 1..2048 directions (steps >= 2)");
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sphere_steps must give 1..1000 directions (steps >= 2)");
   return 0;
 }
 int validate_half_sphere(sfsim_atmo_ctx* ctx, int steps) {
@@ -247,6 +251,17 @@ int ensure_dirs(sfsim_atmo_ctx* ctx, int half, int steps) {
   return 0;
 }
 
+// Direction geometry of the sphere quadrature and the slab scratch of the point-scatter kernels.
+int prepare_point_scatter(sfsim_atmo_ctx* ctx, const DevParams& P, bool with_mie, WorkCounters* wc) {
+  size_t slab = (size_t)P.nS[0] * ctx->sphere_n * P.nS[2] * P.nS[3] * sizeof(float4);
+  TRY(ensure(ctx, ctx->geom, direction_geometry_bytes(P, ctx->sphere_n)));
+  TRY(ensure(ctx, ctx->slab, slab));
+  if (with_mie) TRY(ensure(ctx, ctx->slab_mie, slab));
+  HIP_TRY(ctx, launch_direction_geometry(P, (const Dir*)ctx->sphere_dirs.ptr, ctx->sphere_n, ctx->geom.ptr, wc,
+                                         ctx->stream));
+  return 0;
+}
+
 int activate(sfsim_atmo_ctx* ctx) {
   if (!ctx) return SFSIM_ATMO_ERR_ARGUMENT;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -337,6 +352,11 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), wc, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), wc, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream));
+  if (p->iterations > 0) {
+    TRY(timer.begin(kDirectionGeometry));
+    TRY(prepare_point_scatter(ctx, P, true, wc));
+    TRY(timer.end());
+  }
   // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
   const float4* S_cur = f4(ctx->R1);
   const float4* E_cur = nullptr;
@@ -347,7 +367,8 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
     const float4* mie = first ? f4(ctx->M1) : nullptr;
     // the partials of atmosphere_lut.clj:88-89 bind the previous dS and dE
     LAUNCH(timer, kPointScatter,
-           launch_point_scatter(P, sdirs, ctx->sphere_n, dS_tab, mie, f4(ctx->dE[dE_cur]), f4(ctx->dJ), wc, ctx->stream));
+           launch_point_scatter(P, sdirs, ctx->sphere_n, ctx->geom.ptr, dS_tab, mie, f4(ctx->dE[dE_cur]), f4(ctx->slab),
+                                f4(ctx->slab_mie), f4(ctx->dJ), wc, ctx->stream));
     const int dE_new = 1 - dE_cur;
     LAUNCH(timer, kSurfaceRadiance,
            launch_surface_radiance(P, hdirs, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
@@ -451,7 +472,8 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
                           &ctx->R1,     &ctx->M1,      &ctx->dJ,          &ctx->dS[0],       &ctx->dS[1],
                           &ctx->Sacc[0], &ctx->Sacc[1], &ctx->out[0],     &ctx->out[1],      &ctx->out[2],
                           &ctx->out[3], &ctx->stage_in[0], &ctx->stage_in[1], &ctx->stage_in[2], &ctx->stage_out4,
-                          &ctx->stage_rgb, &ctx->sphere_dirs, &ctx->half_dirs, &ctx->counters};
+                          &ctx->stage_rgb, &ctx->sphere_dirs, &ctx->half_dirs, &ctx->counters,
+                          &ctx->geom,   &ctx->slab,    &ctx->slab_mie};
   for (DeviceBuffer* b : bufs) release(*b);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -520,8 +542,9 @@ int sfsim_atmo_point_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params*
   int rc;
   float4* o = stage_out(ctx, n, &rc);
   TRY(rc);
-  HIP_TRY(ctx, launch_point_scatter(P, (const Dir*)ctx->sphere_dirs.ptr, ctx->sphere_n, ddS, dM, ddE, o, nullptr,
-                                    ctx->stream));
+  TRY(prepare_point_scatter(ctx, P, dM != nullptr, nullptr));
+  HIP_TRY(ctx, launch_point_scatter(P, (const Dir*)ctx->sphere_dirs.ptr, ctx->sphere_n, ctx->geom.ptr, ddS, dM, ddE,
+                                    (float4*)ctx->slab.ptr, (float4*)ctx->slab_mie.ptr, o, nullptr, ctx->stream));
   return download_table(ctx, o, out, n);
 }
 

@@ -6,7 +6,7 @@
 namespace sfsim {
 
 constexpr int kMaxSteps = 256;   // ray_steps limit (LDS arrays of the per-ray kernels)
-constexpr int kMaxSphereDirs = 2048;
+constexpr int kMaxSphereDirs = 1000;  // 64 bytes of LDS per direction in the point-scatter kernel
 
 struct DevParams {
   Geo g;

@@ -329,13 +329,21 @@ __global__ void __launch_bounds__(kBlock) single_scatter_kernel(const DevParams 
   count(wc, samples, samples + view, 0, 0);
 }
 
-// dS' = ray-scatter of an interpolated dJ table (atmosphere_lut.clj:92,94; atmosphere.clj:192-200)
+// dS' = ray-scatter of an interpolated dJ table (atmosphere_lut.clj:92,94; atmosphere.clj:192-200).
+// One workgroup per view ray; lanes over (sun elevation, heading) entries, up to EPT per lane.
+// At sample p_i the height and elevation coordinates of the dJ lookup are the same for every
+// entry of the ray, so the workgroup first folds those two dimensions into a (sun elevation,
+// heading) slab in LDS (coalesced reads of four dJ slabs), Q samples at a time; each lane then
+// interpolates its own sun-elevation / heading coordinates in that slab.
+template <int EPT>
 __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
-                                                             float4* __restrict__ out, WorkCounters* wc) {
+                                                             float4* __restrict__ out, int Q, WorkCounters* wc) {
   __shared__ float4 s_w[kMaxSteps];  // T(x, p_i) rgb, w: height coordinate of p_i
   __shared__ float2 s_g[kMaxSteps];  // x: elevation coordinate of v at p_i, y: t_i
   __shared__ float s_rinv[kMaxSteps];
+  extern __shared__ float4 s_slab[];  // Q x (NL x NA)
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int NLA = NL * NA;
   const int n = P.ray_steps;
   const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
   const Ray ray = make_ray(P, ih, ie);
@@ -349,92 +357,194 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
     s_g[i] = make_float2((float)elevation_to_index(P.g, NE, r, ray.pd0 + t, ray.above), (float)t);
     s_rinv[i] = (float)(1.0 / r);
   }
-  __syncthreads();
-  const long long base = (long long)blockIdx.x * NL * NA;
-  unsigned long long lookups = 0;
-  for (int k = threadIdx.x; k < NL * NA; k += blockDim.x) {
-    int is = k / NA, ia = k - is * NA;
-    double sin_s = index_to_sin_sun_elevation(NL, is);
-    double l0, l1, l2;
-    index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
-    float vl = (float)(ray.se * l0 + ray.ce * l1);
-    float xl = (float)(ray.r0 * l0);
-    Lerp La = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
-    float3 acc = make_float3(0.f, 0.f, 0.f);
-    for (int i = 0; i < n; ++i) {
-      float4 w = s_w[i];
-      float2 gi = s_g[i];
-      float sin_e = fmaf(gi.y, vl, xl) * s_rinv[i];
-      float3 v = lookup4(dJ, NE, NL, NA, make_lerp(w.w, NH), make_lerp(gi.x, NE),
-                         make_lerp(sun_elevation_coord(P, NL, sin_e), NL), La);
-      acc.x += w.x * v.x; acc.y += w.y * v.y; acc.z += w.z * v.z;
+  // per-entry constants: heading coordinate and the terms of dot(p_i, l)
+  float vl[EPT], xl[EPT];
+  Lerp La[EPT];
+  float3 acc[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    int k = threadIdx.x + q * blockDim.x;
+    acc[q] = make_float3(0.f, 0.f, 0.f);
+    vl[q] = 0.f;
+    xl[q] = 0.f;
+    La[q] = make_lerp(0.f, NA);
+    if (k < NLA) {
+      int is = k / NA, ia = k - is * NA;
+      double sin_s = index_to_sin_sun_elevation(NL, is);
+      double l0, l1, l2;
+      index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
+      vl[q] = (float)(ray.se * l0 + ray.ce * l1);
+      xl[q] = (float)(ray.r0 * l0);
+      La[q] = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
     }
-    lookups += n;
-    float a = (float)dt;
-    out[base + k] = make_float4(acc.x * a, acc.y * a, acc.z * a, 0.f);
+  }
+  __syncthreads();
+  const long long sh = (long long)NE * NLA;  // stride of one height slab
+  for (int i0 = 0; i0 < n; i0 += Q) {
+    const int qn = min(Q, n - i0);
+    for (int idx = threadIdx.x; idx < qn * NLA; idx += blockDim.x) {
+      int q = idx / NLA, kk = idx - q * NLA;
+      int i = i0 + q;
+      Lerp Lh = make_lerp(s_w[i].w, NH), Le = make_lerp(s_g[i].x, NE);
+      const float4* b0 = dJ + Lh.i0 * sh + kk;
+      const float4* b1 = dJ + Lh.i1 * sh + kk;
+      float3 u = mix3(ld3(b0, (long long)Le.i0 * NLA), ld3(b0, (long long)Le.i1 * NLA), Le.s);
+      float3 v = mix3(ld3(b1, (long long)Le.i0 * NLA), ld3(b1, (long long)Le.i1 * NLA), Le.s);
+      float3 c = mix3(u, v, Lh.s);
+      s_slab[idx] = make_float4(c.x, c.y, c.z, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      if (threadIdx.x + e * blockDim.x < NLA) {
+        for (int q = 0; q < qn; ++q) {
+          int i = i0 + q;
+          float4 w = s_w[i];
+          float sin_e = fmaf(s_g[i].y, vl[e], xl[e]) * s_rinv[i];
+          float3 v = lookup2(s_slab + q * NLA, NA, make_lerp(sun_elevation_coord(P, NL, sin_e), NL), La[e]);
+          acc[e].x += w.x * v.x; acc[e].y += w.y * v.y; acc[e].z += w.z * v.z;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const long long base = (long long)blockIdx.x * NLA;
+  const float a = (float)dt;
+  unsigned long long lookups = 0;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    int k = threadIdx.x + e * blockDim.x;
+    if (k < NLA) {
+      out[base + k] = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
+      lookups += n;
+    }
   }
   unsigned long long view = (threadIdx.x < n) ? (unsigned long long)n * ((n - threadIdx.x + blockDim.x - 1) / blockDim.x) : 0;
   count(wc, 0, view, lookups, 0);
 }
 
 // dJ = point-scatter over the sphere quadrature (atmosphere_lut.clj:88,91; atmosphere.clj:203-222).
-// One workgroup per view ray (x, v); per-direction data in LDS, lanes over (sun elevation, heading).
-struct DirData {
+// Every grid point of a height index shares its position and quadrature frame, so the per-direction
+// geometry (ray extremity P, surface flag, T(x, P), elevation coordinate of omega at x) depends only
+// on (height, direction) and is computed once per build (direction_geometry_kernel).  The dS lookup
+// at (x, omega, l) has height and elevation coordinates that depend only on (height, direction) as
+// well: point_scatter_combine_kernel folds them into a (sun elevation, heading) slab per
+// (height, direction) once per iteration, shared by every view ray of that height.
+struct DirGeom {
   float4 omega;   // direction xyz, w: elevation coordinate of omega at x (flag: not surface)
   float4 ground;  // T(x, P) * brightness / pi (0 for sky directions), w: 1 for surface directions
   float4 phat;    // P / |P|, w: height coordinate of P in the surface-radiance table
-  float4 weight;  // sum_c beta_c(h(x)) phase_c(v . omega) times the quadrature weight, rgb
+};
+
+__global__ void __launch_bounds__(kBlock) direction_geometry_kernel(const DevParams P, const Dir* __restrict__ dirs,
+                                                                    int ndirs, DirGeom* __restrict__ geom,
+                                                                    WorkCounters* wc) {
+  const int NH = P.nS[0], NE = P.nS[1];
+  const int n = P.ray_steps;
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = idx < NH * ndirs;
+  int ih = active ? idx / ndirs : 0, j = active ? idx - ih * ndirs : 0;
+  const double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
+  Dir d = dirs[j];
+  double pd = r0 * d.x;  // dot(x, omega)
+  bool above = is_above_horizon(P.g, h0, pd);
+  double D = above ? top_exit_distance(P.g, h0, pd) : ground_distance(P.g, h0, pd);
+  double hP = altitude_along(P.g, h0, pd, D);
+  bool surface = 2.0 * hP < P.g.H;  // atmosphere.clj:88-92
+  // every lane evaluates the transmittance (uniform control flow for the wavefront vote inside);
+  // sky directions discard it
+  float3 T = segment_transmittance(P, h0, pd, D, n);
+  if (!active) return;
+  DirGeom g;
+  g.omega = make_float4((float)d.x, (float)d.y, (float)d.z, (float)elevation_to_index(P.g, NE, r0, pd, !surface));
+  if (surface) {
+    g.ground = make_float4(T.x * P.brightness_over_pi[0], T.y * P.brightness_over_pi[1], T.z * P.brightness_over_pi[2],
+                           1.f);
+    double px = r0 + d.x * D, py = d.y * D, pz = d.z * D;
+    double pn = 1.0 / sqrt(px * px + py * py + pz * pz);
+    g.phat = make_float4((float)(px * pn), (float)(py * pn), (float)(pz * pn), (float)altitude_to_index(P.g, P.nE[0], hP));
+  } else {
+    g.ground = make_float4(0.f, 0.f, 0.f, 0.f);
+    g.phat = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  geom[idx] = g;
+  count(wc, 0, surface ? n : 0, 0, 0);
+}
+
+// slab[h][j][s][a] = dS at (height of x_h, elevation coordinate of omega_j) for every (s, a) grid
+// point; with mie != nullptr the Mie strength slab is folded the same way into slab_mie.
+__global__ void __launch_bounds__(kBlock) point_scatter_combine_kernel(const DevParams P, const DirGeom* __restrict__ geom,
+                                                                       int ndirs, const float4* __restrict__ dS,
+                                                                       const float4* __restrict__ mie,
+                                                                       float4* __restrict__ slab,
+                                                                       float4* __restrict__ slab_mie) {
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int NLA = NL * NA;
+  const int ih = blockIdx.x / ndirs;
+  const double r0 = index_to_radius(P.g, NH, ih);
+  const Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
+  const Lerp Le = make_lerp(geom[blockIdx.x].omega.w, NE);
+  const long long sh = (long long)NE * NLA;
+  const long long o00 = Lh.i0 * sh + (long long)Le.i0 * NLA, o01 = Lh.i0 * sh + (long long)Le.i1 * NLA;
+  const long long o10 = Lh.i1 * sh + (long long)Le.i0 * NLA, o11 = Lh.i1 * sh + (long long)Le.i1 * NLA;
+  float4* dst = slab + (long long)blockIdx.x * NLA;
+  float4* dstm = slab_mie ? slab_mie + (long long)blockIdx.x * NLA : nullptr;
+  for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
+    float3 c = mix3(mix3(ld3(dS, o00 + k), ld3(dS, o01 + k), Le.s), mix3(ld3(dS, o10 + k), ld3(dS, o11 + k), Le.s), Lh.s);
+    dst[k] = make_float4(c.x, c.y, c.z, 0.f);
+    if (mie) {
+      float3 m = mix3(mix3(ld3(mie, o00 + k), ld3(mie, o01 + k), Le.s), mix3(ld3(mie, o10 + k), ld3(mie, o11 + k), Le.s),
+                      Lh.s);
+      dstm[k] = make_float4(m.x, m.y, m.z, 0.f);
+    }
+  }
+}
+
+// One workgroup per view ray (x, v): quadrature weights sum_c beta_c(h(x)) phase_c(v . omega) w_j
+// and the direction geometry in LDS, lanes over (sun elevation, heading).
+struct DirLds {
+  float4 omega;   // direction xyz, w: 1 for surface directions
+  float4 weight;  // rgb quadrature weight times in-scattering
+  float4 ground;  // T(x, P) brightness / pi, w: height coordinate of P
+  float4 phat;    // P / |P|
 };
 
 __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
-                                                               int ndirs, const float4* __restrict__ dS,
-                                                               const float4* __restrict__ mie,
+                                                               const DirGeom* __restrict__ geom, int ndirs,
+                                                               const float4* __restrict__ slab,
+                                                               const float4* __restrict__ slab_mie,
                                                                const float4* __restrict__ dE, float4* __restrict__ out,
                                                                WorkCounters* wc) {
-  extern __shared__ DirData s_dir[];
+  extern __shared__ DirLds s_dir[];
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
-  const int n = P.ray_steps;
+  const int NLA = NL * NA;
   const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
   const double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
   double se, ce;
   bool above_unused;
   index_to_elevation(P.g, NE, r0, ie, se, ce, above_unused);
   const float bm = (float)exp(-h0 * P.inv_scale_mie), br = (float)exp(-h0 * P.inv_scale_ray);
-  unsigned long long samples = 0;
+  const DirGeom* gh = geom + (long long)ih * ndirs;
   for (int j = threadIdx.x; j < ndirs; j += blockDim.x) {
     Dir d = dirs[j];
-    double pd = r0 * d.x;  // dot(x, omega)
-    bool above = is_above_horizon(P.g, h0, pd);
-    double D = above ? top_exit_distance(P.g, h0, pd) : ground_distance(P.g, h0, pd);
-    double hP = altitude_along(P.g, h0, pd, D);
-    bool surface = 2.0 * hP < P.g.H;  // atmosphere.clj:88-92
-    DirData dd;
-    dd.omega = make_float4((float)d.x, (float)d.y, (float)d.z, (float)elevation_to_index(P.g, NE, r0, pd, !surface));
-    if (surface) {
-      float3 T = segment_transmittance(P, h0, pd, D, n);
-      samples += n;
-      dd.ground = make_float4(T.x * P.brightness_over_pi[0], T.y * P.brightness_over_pi[1],
-                              T.z * P.brightness_over_pi[2], 1.f);
-      double px = r0 + d.x * D, py = d.y * D, pz = d.z * D;
-      double pn = 1.0 / sqrt(px * px + py * py + pz * pz);
-      dd.phat = make_float4((float)(px * pn), (float)(py * pn), (float)(pz * pn),
-                            (float)altitude_to_index(P.g, P.nE[0], hP));
-    } else {
-      dd.ground = make_float4(0.f, 0.f, 0.f, 0.f);
-      dd.phat = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    DirGeom g = gh[j];
     float vw = (float)(se * d.x + ce * d.y);
     float pm = phase_f(P.g_mie, vw) * bm, pr = phase_f(P.g_ray, vw) * br;
     float w = (float)d.w;
-    dd.weight = make_float4((P.sca_mie[0] * pm + P.sca_ray[0] * pr) * w, (P.sca_mie[1] * pm + P.sca_ray[1] * pr) * w,
+    DirLds dl;
+    dl.omega = make_float4(g.omega.x, g.omega.y, g.omega.z, g.ground.w);
+    dl.weight = make_float4((P.sca_mie[0] * pm + P.sca_ray[0] * pr) * w, (P.sca_mie[1] * pm + P.sca_ray[1] * pr) * w,
                             (P.sca_mie[2] * pm + P.sca_ray[2] * pr) * w, 0.f);
-    s_dir[j] = dd;
+    dl.ground = make_float4(g.ground.x, g.ground.y, g.ground.z, g.phat.w);
+    dl.phat = g.phat;
+    s_dir[j] = dl;
   }
   __syncthreads();
-  const Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
-  const long long base = (long long)blockIdx.x * NL * NA;
+  const float4* sh = slab + (long long)ih * ndirs * NLA;
+  const float4* shm = slab_mie ? slab_mie + (long long)ih * ndirs * NLA : nullptr;
+  const long long base = (long long)blockIdx.x * NLA;
   unsigned long long l4 = 0, l2 = 0;
-  for (int k = threadIdx.x; k < NL * NA; k += blockDim.x) {
+  for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
     int is = k / NA, ia = k - is * NA;
     double sin_s = index_to_sin_sun_elevation(NL, is);
     double l0, l1, l2d;
@@ -443,28 +553,28 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
     const Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
     float3 acc = make_float3(0.f, 0.f, 0.f);
     for (int j = 0; j < ndirs; ++j) {
-      const DirData& dd = s_dir[j];
+      const DirLds& dd = s_dir[j];
       float ol = dd.omega.x * lx + dd.omega.y * ly + dd.omega.z * lz;
-      Lerp Le = make_lerp(dd.omega.w, NE);
       Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
-      float3 v = lookup4(dS, NE, NL, NA, Lh, Le, Ls, La);
-      if (mie) {
-        float3 m = lookup4(mie, NE, NL, NA, Lh, Le, Ls, La);
+      float3 v = lookup2(sh + (long long)j * NLA, NA, Ls, La);
+      if (shm) {
+        float3 m = lookup2(shm + (long long)j * NLA, NA, Ls, La);
         float ph = phase_f(P.g_mie, ol);
         v.x += m.x * ph; v.y += m.y * ph; v.z += m.z * ph;
       }
-      if (dd.ground.w != 0.f) {
+      if (dd.omega.w != 0.f) {
         float spl = dd.phat.x * lx + dd.phat.y * ly + dd.phat.z * lz;
-        float3 e = lookup2(dE, P.nE[1], make_lerp(dd.phat.w, P.nE[0]), make_lerp(sun_elevation_coord(P, P.nE[1], spl), P.nE[1]));
+        float3 e = lookup2(dE, P.nE[1], make_lerp(dd.ground.w, P.nE[0]),
+                           make_lerp(sun_elevation_coord(P, P.nE[1], spl), P.nE[1]));
         v.x += dd.ground.x * e.x; v.y += dd.ground.y * e.y; v.z += dd.ground.z * e.z;
         ++l2;
       }
       acc.x += dd.weight.x * v.x; acc.y += dd.weight.y * v.y; acc.z += dd.weight.z * v.z;
     }
-    l4 += (unsigned long long)ndirs * (mie ? 2 : 1);
+    l4 += (unsigned long long)ndirs * (shm ? 2 : 1);
     out[base + k] = make_float4(acc.x, acc.y, acc.z, 0.f);
   }
-  count(wc, 0, samples, l4, l2);
+  count(wc, 0, 0, l4, l2);
 }
 
 // dE' = surface-radiance of dS (atmosphere_lut.clj:89,93; atmosphere.clj:225-230): one workgroup per
@@ -620,19 +730,51 @@ hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* m
   return hipGetLastError();
 }
 
-hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
-                                const float4* dE, float4* out, WorkCounters* wc, hipStream_t stream) {
-  unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
-  size_t shmem = sizeof(DirData) * (size_t)ndirs;
-  hipLaunchKernelGGL(point_scatter_kernel, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, ndirs, dS, mie, dE, out,
-                     wc);
+hipError_t launch_direction_geometry(const DevParams& P, const Dir* dirs, int ndirs, void* geom, WorkCounters* wc,
+                                    hipStream_t stream) {
+  long long n = (long long)P.nS[0] * ndirs;
+  hipLaunchKernelGGL(direction_geometry_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, dirs, ndirs,
+                     (DirGeom*)geom, wc);
   return hipGetLastError();
+}
+
+size_t direction_geometry_bytes(const DevParams& P, int ndirs) { return sizeof(DirGeom) * (size_t)P.nS[0] * ndirs; }
+
+hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
+                                const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
+                                WorkCounters* wc, hipStream_t stream) {
+  unsigned combos = (unsigned)(P.nS[0] * ndirs);
+  hipLaunchKernelGGL(point_scatter_combine_kernel, dim3(combos), dim3(kBlock), 0, stream, P, (const DirGeom*)geom, ndirs,
+                     dS, mie, slab, mie ? slab_mie : nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
+  size_t shmem = sizeof(DirLds) * (size_t)ndirs;
+  hipLaunchKernelGGL(point_scatter_kernel, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, (const DirGeom*)geom, ndirs,
+                     slab, mie ? slab_mie : nullptr, dE, out, wc);
+  return hipGetLastError();
+}
+
+int ray_scatter_chunk(const DevParams& P) {
+  int nla = P.nS[2] * P.nS[3];
+  int q = 2048 / nla;
+  return q < 1 ? 1 : (q > 8 ? 8 : q);
 }
 
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream) {
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
-  hipLaunchKernelGGL(ray_scatter_kernel, dim3(rays), dim3(kBlock), 0, stream, P, dJ, out, wc);
+  int nla = P.nS[2] * P.nS[3];
+  int Q = ray_scatter_chunk(P);
+  size_t shmem = sizeof(float4) * (size_t)Q * nla;
+  if (nla <= kBlock)
+    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, out, Q, wc);
+  else if (nla <= 2 * kBlock)
+    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, out, Q, wc);
+  else if (nla <= 4 * kBlock)
+    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, out, Q, wc);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

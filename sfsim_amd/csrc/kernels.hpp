@@ -12,10 +12,17 @@ hipError_t launch_transmittance(const DevParams& P, float4* out, WorkCounters* w
 hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream);
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
                                  hipStream_t stream);
+// Per-(height, direction) geometry of the sphere quadrature (once per build); geom holds
+// direction_geometry_bytes(P, ndirs) bytes.
+hipError_t launch_direction_geometry(const DevParams& P, const Dir* dirs, int ndirs, void* geom, WorkCounters* wc,
+                                     hipStream_t stream);
+size_t direction_geometry_bytes(const DevParams& P, int ndirs);
 // dS: previous-order scatter table; mie (nullable): first-iteration Mie strength table, added with the
-// Mie phase of the quadrature direction; dE: previous-order surface-radiance table.
-hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
-                                const float4* dE, float4* out, WorkCounters* wc, hipStream_t stream);
+// Mie phase of the quadrature direction; dE: previous-order surface-radiance table.  slab / slab_mie
+// are scratch tables of NH x ndirs x NL x NA float4 (slab_mie used only when mie is given).
+hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
+                                const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
+                                WorkCounters* wc, hipStream_t stream);
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream);
 hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,

@@ -68,9 +68,9 @@ def test_sphere_direction_counts_match_reference_quadrature():
 
 def test_stage_names():
     lib = atm.lib()
-    names = [lib.sfsim_atmo_stage_name(k).decode() for k in range(8)]
-    assert names == ["transmittance", "surface_radiance_base", "single_scatter", "point_scatter", "surface_radiance",
-                     "ray_scatter", "accumulate", "final_resample"]
+    names = [lib.sfsim_atmo_stage_name(k).decode() for k in range(9)]
+    assert names == ["transmittance", "surface_radiance_base", "single_scatter", "direction_geometry", "point_scatter",
+                     "surface_radiance", "ray_scatter", "accumulate", "final_resample"]
     assert lib.sfsim_atmo_stage_name(99) == b""
 
 
