@@ -69,7 +69,7 @@ def cpu_baseline(P, threads):
     tabS = rng.uniform(1e-6, 1e-3, (nS, 3))
     tabE = rng.uniform(1e-3, 1e-1, (nE, 3))
     sub = lambda n, k: np.sort(rng.choice(n, size=min(n, k), replace=False))
-    k4 = 8192
+    k4 = 32768
     stages = [
         ("transmittance", nT, lambda i: oracle.stage_transmittance(OP, i, threads)),
         ("surface_radiance_base", nE, lambda i: oracle.stage_surface_radiance_base(OP, i, threads)),

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -182,6 +183,9 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   }
   P.k_mie = (float)(-log2e / p->mie.scatter_scale);
   P.k_ray = (float)(-log2e / p->rayleigh.scatter_scale);
+  P.k_mie_2R = (float)(-log2e / p->mie.scatter_scale / (2.0 * P.g.R));
+  P.k_ray_2R = (float)(-log2e / p->rayleigh.scatter_scale / (2.0 * P.g.R));
+  P.inv_R2f = (float)(1.0 / (P.g.R * P.g.R));
   P.inv_scale_mie = 1.0 / p->mie.scatter_scale;
   P.inv_scale_ray = 1.0 / p->rayleigh.scatter_scale;
   P.g_mie = (float)p->mie.scatter_g;
@@ -193,7 +197,14 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   }
   for (int d = 0; d < 4; ++d) P.nS[d] = p->scatter_shape[d];
   P.ray_steps = p->ray_steps;
+  // SFSIM_ATMO_GRAZE_HEIGHT (metres, >= 0) overrides the altitude below which a segment's lowest
+  // point selects the FP64 optical-depth path; 0 keeps every segment in FP32.
   P.graze_height = 2000.0;
+  if (const char* env = std::getenv("SFSIM_ATMO_GRAZE_HEIGHT")) {
+    char* end = nullptr;
+    double v = std::strtod(env, &end);
+    if (end != env && std::isfinite(v) && v >= 0.0) P.graze_height = v;
+  }
   P.graze_u = P.graze_height * (2.0 * P.g.R + P.graze_height);
   return P;
 }

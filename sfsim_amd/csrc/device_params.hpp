@@ -15,6 +15,8 @@ struct DevParams {
   float sca_mie[3], sca_ray[3];
   // exp(-h / scale) = exp2(h * k): k = -log2(e) / scale
   float k_mie, k_ray;
+  float k_mie_2R, k_ray_2R;  // k / 2R (altitude series of the FP32 sums)
+  float inv_R2f;             // 1 / R^2
   double inv_scale_mie, inv_scale_ray;
   float g_mie, g_ray;
   float sun_inv_den;  // 1 / (1 - exp(-3.6)) of sun-elevation-to-index (atmosphere.clj:322-326)

@@ -64,22 +64,28 @@ __device__ __forceinline__ Segment make_segment(const DevParams& P, double h0, d
 }
 
 // Sums over the n midpoint nodes of exp(-h / H_mie) and exp(-h / H_rayleigh) with FP32 altitudes;
-// 8-node partial sums, Kahan-compensated across them.
+// 8-node partial sums, Kahan-compensated across them.  The altitude h = u / (R + sqrt(R^2 + u)) is
+// evaluated as (u / 2R) p(u / R^2) with the series p(x) = 2 (sqrt(1 + x) - 1) / x to x^4 (relative
+// truncation < 5e-7 at the largest x = (Rt^2 - R^2) / R^2 = 0.1 the host accepts, < 2e-8 for a
+// 100 km atmosphere), so a node costs FMAs and the two exponentials only; 1 / 2R is folded into
+// the exponent scales km2, kr2.
 __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) {
-  const float R = (float)P.g.R;
-  const float R2 = (float)(P.g.R * P.g.R);
-  const float km = P.k_mie, kr = P.k_ray;
+  const float km = P.k_mie_2R, kr = P.k_ray_2R, inv_R2 = P.inv_R2f;
   float sm = 0.f, sr = 0.f, cm = 0.f, cr = 0.f;
+  float jf = 0.f;
   int j = 0;
   for (; j + 8 <= n; j += 8) {
     float pm = 0.f, pr = 0.f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      float s = fmaf((float)(j + q), g.dt, g.s0);
+      float s = fmaf(jf, g.dt, g.s0);
       float u = fmaf(s, s + g.b, g.c0);
-      float h = u * frcp(R + fsqrt(R2 + u));
-      pm += fexp2(h * km);
-      pr += fexp2(h * kr);
+      float x = u * inv_R2;
+      float p = fmaf(fmaf(fmaf(fmaf(0.0546875f, x, -0.078125f), x, 0.125f), x, -0.25f), x, 1.0f);
+      float up = u * p;
+      pm += fexp2(up * km);
+      pr += fexp2(up * kr);
+      jf += 1.0f;
     }
     kahan_add(sm, cm, pm);
     kahan_add(sr, cr, pr);
@@ -87,11 +93,14 @@ __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) 
   if (j < n) {
     float pm = 0.f, pr = 0.f;
     for (; j < n; ++j) {
-      float s = fmaf((float)j, g.dt, g.s0);
+      float s = fmaf(jf, g.dt, g.s0);
       float u = fmaf(s, s + g.b, g.c0);
-      float h = u * frcp(R + fsqrt(R2 + u));
-      pm += fexp2(h * km);
-      pr += fexp2(h * kr);
+      float x = u * inv_R2;
+      float p = fmaf(fmaf(fmaf(fmaf(0.0546875f, x, -0.078125f), x, 0.125f), x, -0.25f), x, 1.0f);
+      float up = u * p;
+      pm += fexp2(up * km);
+      pr += fexp2(up * kr);
+      jf += 1.0f;
     }
     kahan_add(sm, cm, pm);
     kahan_add(sr, cr, pr);
@@ -273,19 +282,49 @@ __device__ __forceinline__ Ray make_ray(const DevParams& P, int ih, int ie) {
 }
 
 // R1 (Rayleigh with phase) and M1 (Mie strength) in one pass: atmosphere_lut.clj:70-78,
-// atmosphere.clj:154-200.  One workgroup per view ray; lanes over (sun elevation, heading).
-__global__ void __launch_bounds__(kBlock) single_scatter_kernel(const DevParams P, float4* __restrict__ R1,
-                                                                float4* __restrict__ M1, WorkCounters* wc) {
-  __shared__ double s_t[kMaxSteps];
-  __shared__ double s_h[kMaxSteps];
-  __shared__ float4 s_w[kMaxSteps];    // T(x, p_i) rgb, w: exp(-h_i / H_mie)
-  __shared__ float s_bray[kMaxSteps];  // exp(-h_i / H_rayleigh)
-  const int NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+// atmosphere.clj:154-200.  One wavefront per (view ray, group of 64 (sun elevation, heading)
+// entries); groups of high sun elevation (most samples lit) are launched first so that the light
+// groups fill the end of the grid.  The ray's samples are processed in chunks of kSSChunk: each lane
+// marks the samples where its sun is above the horizon, the wavefront compacts the lit
+// (entry, sample) pairs into an LDS list and all 64 lanes take items from it, so lanes never idle on
+// unlit samples; each entry then adds its results in sample order (deterministic sums).
+constexpr int kSSWave = 64;
+constexpr int kSSChunk = 8;
+
+__device__ __forceinline__ int wave_exclusive_scan(int v, int lane) {
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x - v;
+}
+
+__global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams P, float4* __restrict__ R1,
+                                                                 float4* __restrict__ M1, WorkCounters* wc) {
+  extern __shared__ unsigned char s_raw[];
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int NLA = NL * NA;
   const int n = P.ray_steps;
-  const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
+  double* s_t = (double*)s_raw;                  // [n] sample distance along the ray
+  double* s_h = s_t + n;                         // [n] sample altitude
+  float4* s_w = (float4*)(s_h + n);              // [n] T(x, p_i) rgb, w: exp(-h_i / H_mie)
+  float* s_bray = (float*)(s_w + n);             // [n] exp(-h_i / H_rayleigh)
+  float4* s_res = (float4*)(s_bray + ((n + 3) & ~3));  // [64 * kSSChunk] sun transmittance per item
+  double* s_xl = (double*)(s_res + kSSWave * kSSChunk);  // [64] dot(x, l) of each lane's entry
+  double* s_vl = s_xl + kSSWave;                          // [64] dot(v, l)
+  unsigned short* s_item = (unsigned short*)(s_vl + kSSWave);  // lane << 8 | chunk sample
+
+  const int rays = NH * NE;
+  const int groups = (NLA + kSSWave - 1) / kSSWave;
+  const int group = groups - 1 - blockIdx.x / rays;
+  const int rayid = blockIdx.x - (groups - 1 - group) * rays;
+  const int ih = rayid / NE, ie = rayid - ih * NE;
+  const int lane = threadIdx.x;
   const Ray ray = make_ray(P, ih, ie);
   const double dt = ray.D / n;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = lane; i < n; i += kSSWave) {
     double t = (i + 0.5) * dt;
     double h = altitude_along(P.g, ray.h0, ray.pd0, t);
     s_t[i] = t;
@@ -294,38 +333,78 @@ __global__ void __launch_bounds__(kBlock) single_scatter_kernel(const DevParams 
     s_w[i] = make_float4(T.x, T.y, T.z, (float)exp(-h * P.inv_scale_mie));
     s_bray[i] = (float)exp(-h * P.inv_scale_ray);
   }
-  __syncthreads();
-  unsigned long long samples = 0;
-  const long long base = (long long)blockIdx.x * NL * NA;
-  for (int k = threadIdx.x; k < NL * NA; k += blockDim.x) {
+  const int k = group * kSSWave + lane;
+  const bool valid = k < NLA;
+  double vl = 0.0, xl = 0.0;
+  if (valid) {
     int is = k / NA, ia = k - is * NA;
     double sin_s = index_to_sin_sun_elevation(NL, is);
     double l0, l1, l2;
     index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
-    double vl = ray.se * l0 + ray.ce * l1;
-    double xl = ray.r0 * l0;
-    float3 accR = make_float3(0.f, 0.f, 0.f), accM = make_float3(0.f, 0.f, 0.f);
-    for (int i = 0; i < n; ++i) {
+    vl = ray.se * l0 + ray.ce * l1;
+    xl = ray.r0 * l0;
+  }
+  s_xl[lane] = xl;
+  s_vl[lane] = vl;
+  __syncthreads();
+  float3 accR = make_float3(0.f, 0.f, 0.f), accM = make_float3(0.f, 0.f, 0.f);
+  unsigned long long samples = 0;
+  for (int i0 = 0; i0 < n; i0 += kSSChunk) {
+    const int cn = min(kSSChunk, n - i0);
+    unsigned mask = 0;
+    if (valid)
+      for (int c = 0; c < cn; ++c) {
+        int i = i0 + c;
+        if (is_above_horizon(P.g, s_h[i], xl + s_t[i] * vl)) mask |= 1u << c;  // atmosphere.clj:154-160
+      }
+    const int cnt = __popc(mask);
+    const int off = wave_exclusive_scan(cnt, lane);
+    const int total = __shfl(off + cnt, kSSWave - 1, 64);
+    {
+      unsigned m = mask;
+      for (int q = 0; q < cnt; ++q) {
+        int c = __ffs(m) - 1;
+        m &= m - 1;
+        s_item[off + q] = (unsigned short)((lane << 8) | c);
+      }
+    }
+    __syncthreads();
+    for (int it = lane; it < total; it += kSSWave) {
+      unsigned item = s_item[it];
+      int owner = item >> 8, i = i0 + (int)(item & 255);
       double h = s_h[i];
-      double pl = xl + s_t[i] * vl;  // dot(p_i, l)
-      if (!is_above_horizon(P.g, h, pl)) continue;  // no direct sun (atmosphere.clj:154-160)
+      double pl = s_xl[owner] + s_t[i] * s_vl[owner];  // dot(p_i, l)
       double Ds = top_exit_distance(P.g, h, pl);
       float3 Ts = segment_transmittance(P, h, pl, Ds, n);
-      samples += n;
-      float4 w = s_w[i];
-      float br = s_bray[i];
-      float3 q = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
-      accR.x += q.x * br; accR.y += q.y * br; accR.z += q.z * br;
-      accM.x += q.x * w.w; accM.y += q.y * w.w; accM.z += q.z * w.w;
+      s_res[it] = make_float4(Ts.x, Ts.y, Ts.z, 0.f);
     }
+    __syncthreads();
+    {
+      unsigned m = mask;
+      for (int q = 0; q < cnt; ++q) {
+        int c = __ffs(m) - 1;
+        m &= m - 1;
+        float4 Ts = s_res[off + q];
+        float4 w = s_w[i0 + c];
+        float br = s_bray[i0 + c];
+        float3 t3 = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
+        accR.x += t3.x * br; accR.y += t3.y * br; accR.z += t3.z * br;
+        accM.x += t3.x * w.w; accM.y += t3.y * w.w; accM.z += t3.z * w.w;
+      }
+    }
+    samples += (unsigned long long)cnt * n;
+    __syncthreads();
+  }
+  if (valid) {
+    const long long o = (long long)rayid * NLA + k;
     float a = (float)dt;
     float pr = phase_f(P.g_ray, (float)vl) * a;
-    R1[base + k] = make_float4(accR.x * P.sca_ray[0] * P.intensity[0] * pr, accR.y * P.sca_ray[1] * P.intensity[1] * pr,
-                               accR.z * P.sca_ray[2] * P.intensity[2] * pr, 0.f);
-    M1[base + k] = make_float4(accM.x * P.sca_mie[0] * P.intensity[0] * a, accM.y * P.sca_mie[1] * P.intensity[1] * a,
-                               accM.z * P.sca_mie[2] * P.intensity[2] * a, 0.f);
+    R1[o] = make_float4(accR.x * P.sca_ray[0] * P.intensity[0] * pr, accR.y * P.sca_ray[1] * P.intensity[1] * pr,
+                        accR.z * P.sca_ray[2] * P.intensity[2] * pr, 0.f);
+    M1[o] = make_float4(accM.x * P.sca_mie[0] * P.intensity[0] * a, accM.y * P.sca_mie[1] * P.intensity[1] * a,
+                        accM.z * P.sca_mie[2] * P.intensity[2] * a, 0.f);
   }
-  unsigned long long view = (threadIdx.x < n) ? (unsigned long long)n * ((n - threadIdx.x + blockDim.x - 1) / blockDim.x) : 0;
+  unsigned long long view = (lane < n) ? (unsigned long long)n * ((n - lane + kSSWave - 1) / kSSWave) : 0;
   count(wc, samples, samples + view, 0, 0);
 }
 
@@ -726,7 +805,11 @@ hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCou
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
                                  hipStream_t stream) {
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
-  hipLaunchKernelGGL(single_scatter_kernel, dim3(rays), dim3(kBlock), 0, stream, P, rayleigh, mie, wc);
+  unsigned groups = (unsigned)((P.nS[2] * P.nS[3] + kSSWave - 1) / kSSWave);
+  int n = P.ray_steps;
+  size_t shmem = (size_t)n * (2 * sizeof(double) + sizeof(float4)) + sizeof(float) * (size_t)((n + 3) & ~3) +
+                 (sizeof(float4) + sizeof(unsigned short)) * kSSWave * kSSChunk + 2 * sizeof(double) * kSSWave;
+  hipLaunchKernelGGL(single_scatter_kernel, dim3(rays * groups), dim3(kSSWave), shmem, stream, P, rayleigh, mie, wc);
   return hipGetLastError();
 }
 
