@@ -27,7 +27,9 @@ EXPORTS = (
     "sfsim_atmo_ray_scatter_table", "sfsim_atmo_surface_radiance_table", "sfsim_atmo_resample_table",
     "sfsim_atmo_generate_luts", "sfsim_atmo_build", "sfsim_atmo_fetch", "sfsim_atmo_stage_timings",
     "sfsim_atmo_stage_name", "sfsim_atmo_count_work", "sfsim_atmo_write_scatter", "sfsim_atmo_read_scatter",
-    "sfsim_atmo_sphere_directions",
+    "sfsim_atmo_sphere_directions", "sfsim_atmo_surrogate_size", "sfsim_atmo_surrogate_reset",
+    "sfsim_atmo_surrogate_weights", "sfsim_atmo_surrogate_set_weights", "sfsim_atmo_surrogate_eval",
+    "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_order_sums", "sfsim_atmo_surrogate_predict",
 )
 
 
@@ -146,6 +148,18 @@ def lib():
         L.sfsim_atmo_write_scatter.argtypes = [ctypes.c_char_p, fp, ctypes.c_size_t]
         L.sfsim_atmo_read_scatter.argtypes = [ctypes.c_char_p, fp, ctypes.c_size_t]
         L.sfsim_atmo_sphere_directions.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.sfsim_atmo_surrogate_size.argtypes = []
+        L.sfsim_atmo_surrogate_size.restype = ctypes.c_size_t
+        L.sfsim_atmo_surrogate_reset.argtypes = [vp, ctypes.c_ulonglong, ctypes.c_int]
+        L.sfsim_atmo_surrogate_weights.argtypes = [vp, fp, ctypes.c_size_t]
+        L.sfsim_atmo_surrogate_set_weights.argtypes = [vp, fp, ctypes.c_size_t, ctypes.c_int]
+        L.sfsim_atmo_surrogate_eval.argtypes = [vp, fp, fp, ctypes.c_int, fp, fp, fp]
+        L.sfsim_atmo_surrogate_train.argtypes = [vp, pp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong,
+                                                 fp]
+        L.sfsim_atmo_build_seeded.argtypes = [vp, pp, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_int)]
+        L.sfsim_atmo_order_sums.argtypes = [vp, pp, fp, fp, fp]
+        L.sfsim_atmo_surrogate_predict.argtypes = [vp, pp, fp, fp, fp]
         _lib = L
     return _lib
 
@@ -299,6 +313,86 @@ class Atmosphere:
         c = (ctypes.c_ulonglong * 4)()
         self._check(lib().sfsim_atmo_count_work(self._ctx, ctypes.byref(P), c))
         return {"single_scatter_samples": c[0], "extinction_samples": c[1], "lookups4": c[2], "lookups2": c[3]}
+
+
+def surrogate_size():
+    return lib().sfsim_atmo_surrogate_size()
+
+
+def random_atmosphere(rng, base=None):
+    """An atmosphere drawn from the surrogate's training ranges (Rayleigh scale height 6..10 km and
+    coefficient x0.5..2, Mie scale height 0.8..2 km, coefficient 5e-6..5e-5 /m, g 0.6..0.85);
+    ``base`` supplies grid shapes and step counts."""
+    P = base if base is not None else workload("full-5-order")
+    Q = Params()
+    ctypes.pointer(Q)[0] = P
+    s = rng.uniform(0.5, 2.0)
+    Q.rayleigh.scatter_base[:] = [5.8e-6 * s, 13.5e-6 * s, 33.1e-6 * s]
+    Q.rayleigh.scatter_scale = rng.uniform(6000.0, 10000.0)
+    b = rng.uniform(5e-6, 5e-5)
+    Q.mie.scatter_base[:] = [b, b, b]
+    Q.mie.scatter_scale = rng.uniform(800.0, 2000.0)
+    Q.mie.scatter_g = rng.uniform(0.6, 0.85)
+    return Q
+
+
+class Surrogate:
+    """The learned stand-in for scattering orders >= 2 living on a context (sfsim_atmo_surrogate_*)."""
+
+    def __init__(self, ctx, seed=0, batch=8192):
+        self.ctx = ctx
+        ctx._check(lib().sfsim_atmo_surrogate_reset(ctx._ctx, seed, batch))
+        self.batch = batch
+
+    def weights(self):
+        out = np.zeros(surrogate_size(), np.float32)
+        self.ctx._check(lib().sfsim_atmo_surrogate_weights(self.ctx._ctx, _fp(out), out.size))
+        return out
+
+    def set_weights(self, w, trained=True):
+        w = np.ascontiguousarray(w, np.float32)
+        self.ctx._check(lib().sfsim_atmo_surrogate_set_weights(self.ctx._ctx, _fp(w), w.size, int(trained)))
+
+    def eval(self, inputs, targets=None):
+        """(outputs [rows, 16], gradient or None, loss or None) for input rows of 10 values in [0, 1]."""
+        x = np.ascontiguousarray(inputs, np.float32)
+        rows = x.shape[0]
+        out = np.zeros((rows, 16), np.float32)
+        t = None if targets is None else np.ascontiguousarray(targets, np.float32)
+        g = np.zeros(surrogate_size(), np.float32) if t is not None else None
+        loss = np.zeros(1, np.float32)
+        self.ctx._check(lib().sfsim_atmo_surrogate_eval(self.ctx._ctx, _fp(x), _fp(t), rows, _fp(out), _fp(g),
+                                                        _fp(loss) if t is not None else None))
+        return out, g, (float(loss[0]) if t is not None else None)
+
+    def train(self, atmospheres, steps, lr=2e-3, seed=1):
+        arr = (Params * len(atmospheres))(*atmospheres)
+        losses = np.zeros(max(steps, 1), np.float32)
+        self.ctx._check(lib().sfsim_atmo_surrogate_train(self.ctx._ctx, arr, len(atmospheres), steps, lr, seed,
+                                                         _fp(losses)))
+        return losses[:steps]
+
+    def predict(self, P):
+        shape = tuple(P.scatter_shape) + (3,)
+        out = [np.zeros(shape, np.float32) for _ in range(3)]
+        self.ctx._check(lib().sfsim_atmo_surrogate_predict(self.ctx._ctx, ctypes.byref(P), *[_fp(o) for o in out]))
+        return out  # P3, P4, P5
+
+
+def order_sums(ctx, P):
+    """P3, P4, P5: sums of the plain iteration's scattering tables dS_1..dS_k."""
+    shape = tuple(P.scatter_shape) + (3,)
+    out = [np.zeros(shape, np.float32) for _ in range(3)]
+    ctx._check(lib().sfsim_atmo_order_sums(ctx._ctx, ctypes.byref(P), *[_fp(o) for o in out]))
+    return out
+
+
+def build_seeded(ctx, P, tolerance=1e-3):
+    """sfsim_atmo_build_seeded: returns (residual, fell_back)."""
+    r = ctypes.c_double()
+    f = ctypes.c_int()
+    ctx._check(lib().sfsim_atmo_build_seeded(ctx._ctx, ctypes.byref(P), tolerance, ctypes.byref(r), ctypes.byref(f)))
+    return r.value, bool(f.value)
 
 
 def write_scatter_files(directory, luts):

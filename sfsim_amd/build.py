@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsfsim_atmo.so")
-SOURCES = ["kernels.hip", "atmo.cpp"]
+SOURCES = ["kernels.hip", "surrogate.hip", "atmo.cpp"]
 
 # -packed-fp32-ops: no v_pk_*_f32 generated for the device code (hipcc passes the flag to both
 # compilation passes; the host pass warns that it does not know the feature).

@@ -13,6 +13,7 @@
 #include "device_params.hpp"
 #include "geometry.hpp"
 #include "kernels.hpp"
+#include "surrogate.hpp"
 
 using namespace sfsim;
 
@@ -28,12 +29,14 @@ enum Kind {
   kRayScatter,
   kAccumulate,
   kFinal,
+  kSurrogate,
   kNumKinds
 };
 
 const char* const kKindNames[kNumKinds] = {"transmittance",      "surface_radiance_base", "single_scatter",
                                            "direction_geometry", "point_scatter",         "surface_radiance",
-                                           "ray_scatter",        "accumulate",            "final_resample"};
+                                           "ray_scatter",        "accumulate",            "final_resample",
+                                           "surrogate"};
 
 struct DeviceBuffer {
   void* ptr = nullptr;
@@ -59,6 +62,10 @@ struct sfsim_atmo_ctx {
   DeviceBuffer geom, slab, slab_mie;  // point-scatter direction geometry and folded dS slabs
   int sphere_key = -1, half_key = -1, sphere_n = 0, half_n = 0;
   DeviceBuffer counters;
+  // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
+  Surrogate sg;
+  DeviceBuffer codes, targets, sumS, cap[3], pred[3], resid;
+  double last_residual = 0.0;
   // timing
   std::vector<hipEvent_t> events;
   std::vector<int> event_kind;
@@ -328,7 +335,10 @@ struct Timer {
   } while (0)
 
 // The whole driver of atmosphere_lut.clj:43-105 on device buffers of the context.
-int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* wc, bool timed) {
+// capture (nullable): receives the sums dS_1..dS_k of the scattering tables for k = 3, 4, 5 (float4
+// tables), the training targets of the surrogate.
+int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* wc, bool timed,
+                 float4* const* capture = nullptr) {
   TRY(validate(ctx, p));
   if (p->iterations > 0) {
     TRY(validate_sphere(ctx, p->sphere_steps));
@@ -387,6 +397,17 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
     LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[dS_new]), wc, ctx->stream));
     dE_cur = dE_new;
     dS_cur = dS_new;
+    if (capture) {
+      const float4* d = f4(ctx->dS[dS_cur]);
+      float4* sum = f4(ctx->sumS);
+      if (it == 0)
+        HIP_TRY(ctx, hipMemcpyAsync(sum, d, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+      else
+        HIP_TRY(ctx, launch_add4(sum, d, sum, (long long)nS, ctx->stream));
+      const int order = it + 1;  // capture[0] = k 3, capture[1] = k 4, capture[2] = k 5
+      if (order >= 3 && order <= 5)
+        HIP_TRY(ctx, hipMemcpyAsync(capture[order - 3], sum, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+    }
     // E <- table(E + dE), S <- table(S + dS) (atmosphere_lut.clj:95-97)
     LAUNCH(timer, kAccumulate,
            launch_resample(P, 3, f4(ctx->dE[dE_cur]), E_cur, f4(ctx->Eacc[e_next]), nullptr, 0, ctx->stream));
@@ -405,6 +426,110 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
     HIP_TRY(ctx, hipMemsetAsync(ctx->out[1].ptr, 0, outc[1] * sizeof(float), ctx->stream));
   }
   LAUNCH(timer, kFinal, launch_resample(P, 4, S_cur, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
+  LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  TRY(timer.collect());
+  ctx->built = true;
+  return 0;
+}
+
+// ------------------------------------------------------------------ surrogate-seeded build
+AtmosphereCode code_of(const sfsim_atmo_params* p) {
+  return atmosphere_code(p->rayleigh.scatter_base, p->rayleigh.scatter_scale, p->mie.scatter_base[0],
+                         p->mie.scatter_scale, p->mie.scatter_g, p->planet.brightness[0]);
+}
+
+// The 5-order set with the network standing in for orders 2-5 of the state the iteration carries
+// (atmosphere_lut.clj:86-97 is the linear recurrence (dS_k, dE_k) = (S[J[dS_{k-1}, dE_{k-1}]], E[dS_{k-1}])):
+// with P_k = dS_1 + ... + dS_k predicted for k = 3, 4, 5,
+//   sum_{k<=4} dS_k = dS_0 + P4,   sum_{k<=4} dE_k = E0 + E[dS_0 + P3],
+// and one application of the recurrence to these sums gives sum_{k=1..5} (dS_k, dE_k), with the
+// network's error multiplied by the recurrence's (contracting) gain.  The network's own P5 against
+// that result is the residual; above `tol` the plain iteration runs instead.
+int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tol, double* residual, int* fallback) {
+  TRY(validate(ctx, p));
+  if (p->iterations != 5) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "the surrogate stands in for a 5-order set");
+  if (!ctx->sg.trained) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate is not trained");
+  TRY(validate_sphere(ctx, p->sphere_steps));
+  TRY(validate_half_sphere(ctx, p->ray_steps));
+  TRY(ensure_dirs(ctx, 0, p->sphere_steps));
+  TRY(ensure_dirs(ctx, 1, p->ray_steps));
+  const DevParams P = make_dev_params(p);
+  const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  TRY(ensure(ctx, ctx->T, nT * sizeof(float4)));
+  for (int i = 0; i < 2; ++i) {
+    TRY(ensure(ctx, ctx->dE[i], nE * sizeof(float4)));
+    TRY(ensure(ctx, ctx->Eacc[i], nE * sizeof(float4)));
+    TRY(ensure(ctx, ctx->dS[i], nS * sizeof(float4)));
+    TRY(ensure(ctx, ctx->Sacc[i], nS * sizeof(float4)));
+  }
+  TRY(ensure(ctx, ctx->R1, nS * sizeof(float4)));
+  TRY(ensure(ctx, ctx->M1, nS * sizeof(float4)));
+  TRY(ensure(ctx, ctx->dJ, nS * sizeof(float4)));
+  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
+  TRY(ensure(ctx, ctx->resid, 4 * sizeof(unsigned)));
+  const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
+  for (int i = 0; i < 4; ++i) {
+    TRY(ensure(ctx, ctx->out[i], outc[i] * sizeof(float)));
+    ctx->out_count[i] = outc[i];
+  }
+  ctx->built = false;
+  Timer timer{ctx, true};
+  auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
+  const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
+  const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
+  const long long n4 = (long long)nS;
+  LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), nullptr, ctx->stream));
+  LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), nullptr, ctx->stream));
+  LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), nullptr, ctx->stream));
+  TRY(timer.begin(kDirectionGeometry));
+  TRY(prepare_point_scatter(ctx, P, true, nullptr));
+  TRY(timer.end());
+  // network: P4, P3, P5
+  TRY(timer.begin(kSurrogate));
+  HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), f4(ctx->pred[0]), f4(ctx->pred[1]), f4(ctx->pred[2]),
+                                        ctx->stream));
+  TRY(timer.end());
+  float4* A = f4(ctx->Sacc[0]);  // R1 + P4  (dS_0 + P4 without the Mie phase term)
+  float4* B = f4(ctx->Sacc[1]);  // R1 + P3
+  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->pred[0]), A, n4, ctx->stream));
+  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->pred[1]), B, n4, ctx->stream));
+  // sum_{k<=4} dE_k = E0 + E[dS_0 + P3]
+  LAUNCH(timer, kSurfaceRadiance,
+         launch_surface_radiance(P, hdirs, ctx->half_n, B, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
+  LAUNCH(timer, kAccumulate, launch_resample(P, 3, f4(ctx->dE[0]), f4(ctx->dE[1]), f4(ctx->Eacc[0]), nullptr, 0,
+                                             ctx->stream));
+  // one application: sum_{k=1..5} dS_k = S[J[sum dS, sum dE]], sum_{k=1..5} dE_k = E[sum dS]
+  LAUNCH(timer, kPointScatter,
+         launch_point_scatter(P, sdirs, ctx->sphere_n, ctx->geom.ptr, A, f4(ctx->M1), f4(ctx->Eacc[0]), f4(ctx->slab),
+                              f4(ctx->slab_mie), f4(ctx->dJ), nullptr, ctx->stream));
+  LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[0]), nullptr, ctx->stream));
+  LAUNCH(timer, kSurfaceRadiance,
+         launch_surface_radiance(P, hdirs, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
+  // residual of the network's own 5-order sum, relative to R1 + sum
+  TRY(timer.begin(kSurrogate));
+  HIP_TRY(ctx, launch_residual(f4(ctx->R1), f4(ctx->dS[0]), f4(ctx->pred[2]), n4, 1e-6f, (unsigned*)ctx->resid.ptr,
+                               ctx->stream));
+  TRY(timer.end());
+  unsigned bits[4];
+  HIP_TRY(ctx, hipMemcpyAsync(bits, ctx->resid.ptr, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  float r;
+  std::memcpy(&r, &bits[3], sizeof(r));
+  *residual = r;
+  ctx->last_residual = r;
+  if (!(r <= tol)) {
+    *fallback = 1;
+    return run_pipeline(ctx, p, nullptr, true);
+  }
+  *fallback = 0;
+  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->dS[0]), f4(ctx->dS[1]), n4, ctx->stream));
+  LAUNCH(timer, kFinal, launch_resample(P, 2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, ctx->stream));
+  LAUNCH(timer, kFinal,
+         launch_resample(P, 3, f4(ctx->dE[1]), nullptr, nullptr, (float*)ctx->out[1].ptr, 0, ctx->stream));
+  LAUNCH(timer, kFinal,
+         launch_resample(P, 4, f4(ctx->dS[1]), nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
   LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   TRY(timer.collect());
@@ -486,6 +611,10 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
                           &ctx->stage_rgb, &ctx->sphere_dirs, &ctx->half_dirs, &ctx->counters,
                           &ctx->geom,   &ctx->slab,    &ctx->slab_mie};
   for (DeviceBuffer* b : bufs) release(*b);
+  DeviceBuffer* more[] = {&ctx->codes, &ctx->targets, &ctx->sumS, &ctx->cap[0], &ctx->cap[1], &ctx->cap[2],
+                          &ctx->pred[0], &ctx->pred[1], &ctx->pred[2], &ctx->resid};
+  for (DeviceBuffer* b : more) release(*b);
+  surrogate_free(ctx->sg);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -666,6 +795,168 @@ int sfsim_atmo_count_work(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, unsig
   counts[2] = wc.lookups4;
   counts[3] = wc.lookups2;
   return 0;
+}
+
+// ------------------------------------------------------------------ surrogate entry points
+size_t sfsim_atmo_surrogate_size(void) { return MlpLayout::total; }
+
+int sfsim_atmo_surrogate_reset(sfsim_atmo_ctx* ctx, unsigned long long seed, int batch) {
+  TRY(activate(ctx));
+  if (batch < 64 || batch > (1 << 20) || batch % 64 != 0)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "batch must be a multiple of 64 in [64, 1048576]");
+  HIP_TRY(ctx, surrogate_alloc(ctx->sg, batch));
+  HIP_TRY(ctx, surrogate_init(ctx->sg, seed, ctx->stream));
+  return 0;
+}
+
+int sfsim_atmo_surrogate_weights(sfsim_atmo_ctx* ctx, float* out, size_t count) {
+  TRY(activate(ctx));
+  if (!ctx->sg.theta) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate not initialised");
+  if (!out || count != MlpLayout::total) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "count must be the parameter count");
+  HIP_TRY(ctx, hipMemcpyAsync(out, ctx->sg.theta, count * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sfsim_atmo_surrogate_set_weights(sfsim_atmo_ctx* ctx, const float* in, size_t count, int trained) {
+  TRY(activate(ctx));
+  if (!ctx->sg.theta) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate not initialised");
+  if (!in || count != MlpLayout::total) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "count must be the parameter count");
+  if (!finite_floats(in, count)) return fail(ctx, SFSIM_ATMO_ERR_NONFINITE, "weights hold NaN/Inf");
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->sg.theta, in, count * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, surrogate_sync_bf16(ctx->sg, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->sg.trained = trained != 0;
+  return 0;
+}
+
+int sfsim_atmo_surrogate_eval(sfsim_atmo_ctx* ctx, const float* inputs, const float* targets, int rows,
+                              float* outputs, float* grad, float* loss) {
+  TRY(activate(ctx));
+  Surrogate& sg = ctx->sg;
+  if (!sg.theta) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate not initialised");
+  if (!inputs || rows < 1 || rows > sg.batch_capacity)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "rows must be in [1, batch capacity] and inputs non-NULL");
+  if (!finite_floats(inputs, (size_t)rows * kMlpIn)) return fail(ctx, SFSIM_ATMO_ERR_NONFINITE, "inputs hold NaN/Inf");
+  TRY(ensure(ctx, ctx->stage_rgb, (size_t)rows * kMlpIn * sizeof(float)));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->stage_rgb.ptr, inputs, (size_t)rows * kMlpIn * sizeof(float),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, surrogate_encode_rows(sg, (const float*)ctx->stage_rgb.ptr, rows, ctx->stream));
+  HIP_TRY(ctx, surrogate_forward(sg, rows, ctx->stream));
+  if (outputs)
+    HIP_TRY(ctx, hipMemcpyAsync(outputs, sg.Y, (size_t)rows * kMlpOut * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  if (targets) {
+    HIP_TRY(ctx, hipMemcpyAsync(sg.T, targets, (size_t)rows * kMlpOut * sizeof(float), hipMemcpyHostToDevice,
+                                ctx->stream));
+    HIP_TRY(ctx, surrogate_backward(sg, rows, ctx->stream));
+    if (grad)
+      HIP_TRY(ctx, hipMemcpyAsync(grad, sg.grad, MlpLayout::total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (loss) {
+      int parts = (rows * kMlpOut + 255) / 256;
+      std::vector<float> h(parts);
+      HIP_TRY(ctx, hipMemcpyAsync(h.data(), sg.loss, parts * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      double t = 0;
+      for (float v : h) t += v;
+      *loss = (float)t;
+    }
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atmospheres, int count, int steps,
+                               double lr, unsigned long long seed, float* losses) {
+  TRY(activate(ctx));
+  if (!atmospheres || count < 1 || steps < 0) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "need >= 1 atmosphere");
+  if (!ctx->sg.theta) TRY(sfsim_atmo_surrogate_reset(ctx, seed, 8192));
+  const sfsim_atmo_params& p0 = atmospheres[0];
+  for (int a = 0; a < count; ++a) {
+    TRY(validate(ctx, &atmospheres[a]));
+    if (atmospheres[a].iterations != 5) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "training needs 5-order builds");
+    if (std::memcmp(atmospheres[a].scatter_shape, p0.scatter_shape, sizeof(p0.scatter_shape)) != 0)
+      return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "all training atmospheres need the same scatter shape");
+  }
+  const size_t nS = (size_t)count4(p0.scatter_shape);
+  TRY(ensure(ctx, ctx->sumS, nS * sizeof(float4)));
+  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->cap[i], nS * sizeof(float4)));
+  TRY(ensure(ctx, ctx->targets, (size_t)count * nS * kMlpUsed * sizeof(float)));
+  TRY(ensure(ctx, ctx->codes, (size_t)count * sizeof(AtmosphereCode)));
+  std::vector<AtmosphereCode> codes(count);
+  // training data: the plain GPU iteration of every atmosphere (orders 1..5, sums k = 3, 4, 5)
+  for (int a = 0; a < count; ++a) {
+    float4* caps[3] = {(float4*)ctx->cap[0].ptr, (float4*)ctx->cap[1].ptr, (float4*)ctx->cap[2].ptr};
+    TRY(run_pipeline(ctx, &atmospheres[a], nullptr, false, caps));
+    HIP_TRY(ctx, surrogate_make_targets(caps[1], caps[0], caps[2], (long long)nS,
+                                        (float*)ctx->targets.ptr + (size_t)a * nS * kMlpUsed, ctx->stream));
+    codes[a] = code_of(&atmospheres[a]);
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->codes.ptr, codes.data(), count * sizeof(AtmosphereCode), hipMemcpyHostToDevice,
+                              ctx->stream));
+  const DevParams P = make_dev_params(&p0);
+  Surrogate& sg = ctx->sg;
+  const int rows = sg.batch_capacity;
+  const int parts = (rows * kMlpOut + 255) / 256;
+  std::vector<float> part(parts);
+  for (int step = 0; step < steps; ++step) {
+    float lr_t = (float)(lr * 0.5 * (1.0 + std::cos(M_PI * step / steps)));
+    HIP_TRY(ctx, surrogate_gather_batch(sg, P, (const AtmosphereCode*)ctx->codes.ptr, count,
+                                        (const float*)ctx->targets.ptr, seed, step, rows, ctx->stream));
+    HIP_TRY(ctx, surrogate_forward(sg, rows, ctx->stream));
+    HIP_TRY(ctx, surrogate_backward(sg, rows, ctx->stream));
+    HIP_TRY(ctx, surrogate_adam(sg, lr_t, ctx->stream));
+    if (losses) {
+      HIP_TRY(ctx, hipMemcpyAsync(part.data(), sg.loss, parts * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      double t = 0;
+      for (float v : part) t += v;
+      losses[step] = (float)t;
+    }
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  sg.trained = true;
+  ctx->built = false;
+  return 0;
+}
+
+int sfsim_atmo_build_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tolerance, double* residual,
+                            int* fallback) {
+  TRY(activate(ctx));
+  double r = 0;
+  int f = 0;
+  TRY(run_seeded(ctx, p, tolerance, &r, &f));
+  if (residual) *residual = r;
+  if (fallback) *fallback = f;
+  return 0;
+}
+
+int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* p3, float* p4, float* p5) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (p->iterations != 5) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "order sums need a 5-order build");
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  TRY(ensure(ctx, ctx->sumS, nS * sizeof(float4)));
+  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->cap[i], nS * sizeof(float4)));
+  float4* caps[3] = {(float4*)ctx->cap[0].ptr, (float4*)ctx->cap[1].ptr, (float4*)ctx->cap[2].ptr};
+  TRY(run_pipeline(ctx, p, nullptr, false, caps));
+  TRY(download_table(ctx, caps[0], p3, nS));
+  TRY(download_table(ctx, caps[1], p4, nS));
+  return download_table(ctx, caps[2], p5, nS);
+}
+
+int sfsim_atmo_surrogate_predict(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* p3, float* p4, float* p5) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (!ctx->sg.theta) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate not initialised");
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
+  const DevParams P = make_dev_params(p);
+  HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), (float4*)ctx->pred[0].ptr, (float4*)ctx->pred[1].ptr,
+                                        (float4*)ctx->pred[2].ptr, ctx->stream));
+  TRY(download_table(ctx, (const float4*)ctx->pred[1].ptr, p3, nS));
+  TRY(download_table(ctx, (const float4*)ctx->pred[0].ptr, p4, nS));
+  return download_table(ctx, (const float4*)ctx->pred[2].ptr, p5, nS);
 }
 
 int sfsim_atmo_write_scatter(const char* path, const float* data, size_t count) {

@@ -786,6 +786,53 @@ __global__ void unpack_rgb_kernel(const float* __restrict__ in, float4* __restri
   out[i] = make_float4(in[i * 3], in[i * 3 + 1], in[i * 3 + 2], 0.f);
 }
 
+__global__ void add4_kernel(const float4* __restrict__ a, const float4* __restrict__ b, float4* __restrict__ c,
+                            long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float4 x = a[i], y = b[i];
+  c[i] = make_float4(x.x + y.x, x.y + y.y, x.z + y.z, 0.f);
+}
+
+// Non-negative float maxima as ordered unsigned integers (atomicMax on the bit patterns).
+__global__ void channel_max_kernel(const float4* __restrict__ a, const float4* __restrict__ b, long long count,
+                                   unsigned* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float3 m = make_float3(0.f, 0.f, 0.f);
+  if (i < count) {
+    float4 x = a[i], y = b[i];
+    m = make_float3(fabsf(x.x + y.x), fabsf(x.y + y.y), fabsf(x.z + y.z));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m.x = fmaxf(m.x, __shfl_down(m.x, off, 64));
+    m.y = fmaxf(m.y, __shfl_down(m.y, off, 64));
+    m.z = fmaxf(m.z, __shfl_down(m.z, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out + 0, __float_as_uint(m.x));
+    atomicMax(out + 1, __float_as_uint(m.y));
+    atomicMax(out + 2, __float_as_uint(m.z));
+  }
+}
+
+// max over entries and channels of |x - y| / max(|a + x|, floor_c), floor_c = fraction * channel max.
+__global__ void residual_kernel(const float4* __restrict__ a, const float4* __restrict__ x, const float4* __restrict__ y,
+                                long long count, const unsigned* __restrict__ cmax, float fraction,
+                                unsigned* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float r = 0.f;
+  if (i < count) {
+    float4 av = a[i], xv = x[i], yv = y[i];
+    float f0 = fraction * __uint_as_float(cmax[0]), f1 = fraction * __uint_as_float(cmax[1]),
+          f2 = fraction * __uint_as_float(cmax[2]);
+    r = fmaxf(fabsf(xv.x - yv.x) / fmaxf(fabsf(av.x + xv.x), fmaxf(f0, 1e-30f)),
+              fmaxf(fabsf(xv.y - yv.y) / fmaxf(fabsf(av.y + xv.y), fmaxf(f1, 1e-30f)),
+                    fabsf(xv.z - yv.z) / fmaxf(fabsf(av.z + xv.z), fmaxf(f2, 1e-30f))));
+  }
+  for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_down(r, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(r));
+}
+
 inline unsigned blocks_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -876,6 +923,23 @@ hipError_t launch_resample(const DevParams& P, int space, const float4* a, const
                              : (long long)P.nE[0] * P.nE[1];
   hipLaunchKernelGGL(resample_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, space, a, b, out4, out_rgb,
                      tiled);
+  return hipGetLastError();
+}
+
+hipError_t launch_add4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream) {
+  hipLaunchKernelGGL(add4_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, b, c, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const float4* a, const float4* x, const float4* y, long long count, float fraction,
+                           unsigned* scratch4, hipStream_t stream) {
+  hipError_t e = hipMemsetAsync(scratch4, 0, 4 * sizeof(unsigned), stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(channel_max_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, x, count, scratch4);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, x, y, count, scratch4,
+                     fraction, scratch4 + 3);
   return hipGetLastError();
 }
 

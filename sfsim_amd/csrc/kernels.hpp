@@ -32,6 +32,12 @@ hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndir
 // tiled != 0 the 4D->2D tiling of the .scatter files.
 hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
                            float* out_rgb, int tiled, hipStream_t stream);
+// c = a + b (float4 tables)
+hipError_t launch_add4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream);
+// scratch4[3] <- bits of max_i max_c |x - y| / max(|a + x|, fraction * max_i |a + x|_c) (scratch4[0..2]:
+// the channel maxima), all as float bit patterns.
+hipError_t launch_residual(const float4* a, const float4* x, const float4* y, long long count, float fraction,
+                           unsigned* scratch4, hipStream_t stream);
 // Pack a float4 grid-order table into RGB floats (grid order), for the stage entry points.
 hipError_t launch_pack_rgb(const float4* in, float* out, long long count, hipStream_t stream);
 hipError_t launch_unpack_rgb(const float* in, float4* out, long long count, hipStream_t stream);

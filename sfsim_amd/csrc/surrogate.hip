@@ -1,0 +1,429 @@
+// MLP surrogate kernels: bf16 matrix-core products with FP32 accumulation, Fourier features, SiLU,
+// loss, bias gradients and Adam, plus the batch gather of training data and table prediction.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "surrogate.hpp"
+
+namespace sfsim {
+
+namespace {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kEw = 256;  // elementwise block size
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+__device__ __forceinline__ float dsilu(float z) {
+  float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// One wavefront computes a 32 x 32 tile of C with v_mfma_f32_32x32x8bf16_1k.  Operand layout of that
+// instruction: lane l supplies A(l % 32, 4 (l / 32) + i) and B(4 (l / 32) + i, l % 32), i = 0..3;
+// accumulator element e of lane l is C(8 (e / 4) + 4 (l / 32) + e % 4, l % 32).
+__global__ void __launch_bounds__(64) gemm_kernel(const uint16_t* __restrict__ A, long long sam, long long sak,
+                                                  const uint16_t* __restrict__ B, long long sbk, long long sbn,
+                                                  float* __restrict__ C, int ldc, const float* __restrict__ bias,
+                                                  int act, uint16_t* __restrict__ outb, int M, int N, int K) {
+  const int lane = threadIdx.x;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int r = lane & 31, kq = (lane >> 5) * 4;
+  const int am = m0 + r, bn = n0 + r;
+  const bool arow = am < M, bcol = bn < N;
+  float16_t acc;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) acc[e] = 0.f;
+  for (int k0 = 0; k0 < K; k0 += 8) {
+    short4_t va, vb;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int k = k0 + kq + i;
+      bool kin = k < K;
+      va[i] = (short)((arow && kin) ? A[am * sam + k * sak] : 0);
+      vb[i] = (short)((bcol && kin) ? B[k * sbk + bn * sbn] : 0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(va, vb, acc, 0, 0, 0);
+  }
+  const int n = n0 + r;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    int m = m0 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+    if (m < M && n < N) {
+      float v = acc[e] + (bias ? bias[n] : 0.f);
+      long long o = (long long)m * ldc + n;
+      if (C) C[o] = v;
+      if (outb) outb[o] = f2bf(act == 1 ? silu(v) : v);
+    }
+  }
+}
+
+// Fourier features of one input row: x, then sin / cos (2^b pi x) for b = 0..7, per input; zero pad.
+__device__ inline void encode_row(const float* x, uint16_t* out) {
+  int f = 0;
+  for (int i = 0; i < kMlpIn; ++i) {
+    float v = x[i];
+    out[f++] = f2bf(v);
+    float a = 3.14159265358979f * v;
+    for (int b = 0; b < 8; ++b) {
+      float sn, cs;
+      sincosf(a, &sn, &cs);
+      out[f++] = f2bf(sn);
+      out[f++] = f2bf(cs);
+      a *= 2.0f;
+    }
+  }
+  for (; f < kMlpFeat; ++f) out[f] = 0;
+}
+
+__global__ void encode_rows_kernel(const float* __restrict__ in, uint16_t* __restrict__ X, int rows) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  encode_row(in + (long long)r * kMlpIn, X + (long long)r * kMlpFeat);
+}
+
+__device__ inline void entry_coords(const DevParams& P, long long entry, float* x) {
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  int ia = (int)(entry % NA);
+  long long q = entry / NA;
+  int is = (int)(q % NL);
+  q /= NL;
+  int ie = (int)(q % NE);
+  int ih = (int)(q / NE);
+  x[0] = (float)ih / (NH - 1);
+  x[1] = (float)ie / (NE - 1);
+  x[2] = (float)is / (NL - 1);
+  x[3] = (float)ia / (NA - 1);
+}
+
+__global__ void gather_kernel(const DevParams P, const AtmosphereCode* __restrict__ codes, int natm,
+                              const float* __restrict__ targets, uint64_t seed, int step, int rows,
+                              uint16_t* __restrict__ X, float* __restrict__ T) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const long long entries = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
+  uint64_t h = splitmix64(seed ^ splitmix64(((uint64_t)step << 32) | (uint64_t)r));
+  int atm = (int)(h % (uint64_t)natm);
+  long long entry = (long long)((h >> 16) % (uint64_t)entries);
+  float x[kMlpIn];
+  entry_coords(P, entry, x);
+  for (int i = 0; i < 6; ++i) x[4 + i] = codes[atm].v[i];
+  encode_row(x, X + (long long)r * kMlpFeat);
+  const float* t = targets + ((long long)atm * entries + entry) * kMlpUsed;
+  for (int o = 0; o < kMlpOut; ++o) T[(long long)r * kMlpOut + o] = o < kMlpUsed ? t[o] : 0.f;
+}
+
+__global__ void encode_entries_kernel(const DevParams P, AtmosphereCode code, long long first, int rows,
+                                      uint16_t* __restrict__ X) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float x[kMlpIn];
+  entry_coords(P, first + r, x);
+  for (int i = 0; i < 6; ++i) x[4 + i] = code.v[i];
+  encode_row(x, X + (long long)r * kMlpFeat);
+}
+
+__global__ void decode_kernel(const float* __restrict__ Y, long long first, int rows, float4* __restrict__ p4,
+                              float4* __restrict__ p3, float4* __restrict__ p5) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const float* y = Y + (long long)r * kMlpOut;
+  float v[kMlpUsed];
+  for (int o = 0; o < kMlpUsed; ++o) v[o] = fmaxf(expf(y[o] - kMlpShift) - kMlpEps, 0.f);
+  p4[first + r] = make_float4(v[0], v[1], v[2], 0.f);
+  p3[first + r] = make_float4(v[3], v[4], v[5], 0.f);
+  p5[first + r] = make_float4(v[6], v[7], v[8], 0.f);
+}
+
+__global__ void targets_kernel(const float4* __restrict__ p4, const float4* __restrict__ p3,
+                               const float4* __restrict__ p5, long long entries, float* __restrict__ t) {
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= entries) return;
+  float4 a = p4[e], b = p3[e], c = p5[e];
+  float v[kMlpUsed] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+  for (int o = 0; o < kMlpUsed; ++o) t[e * kMlpUsed + o] = logf(fmaxf(v[o], 0.f) + kMlpEps) + kMlpShift;
+}
+
+// dY = mask (Y - T) / rows (FP32 and bf16), loss partials 0.5 sum d^2 / rows per block.
+__global__ void __launch_bounds__(kEw) loss_kernel(const float* __restrict__ Y, const float* __restrict__ T, int rows,
+                                                   float* __restrict__ dYf, uint16_t* __restrict__ dY,
+                                                   float* __restrict__ partial) {
+  __shared__ float red[kEw / 64];
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < (long long)rows * kMlpOut) {
+    int o = (int)(i % kMlpOut);
+    float d = o < kMlpUsed ? (Y[i] - T[i]) : 0.f;
+    l = 0.5f * d * d / rows;
+    float g = d / rows;
+    dYf[i] = g;
+    dY[i] = f2bf(g);
+  }
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (int w = 0; w < kEw / 64; ++w) s += red[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+__global__ void dsilu_kernel(const float* __restrict__ dA, const float* __restrict__ Z, long long n,
+                             float* __restrict__ dZf, uint16_t* __restrict__ dZ) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float g = dA[i] * dsilu(Z[i]);
+  dZf[i] = g;
+  dZ[i] = f2bf(g);
+}
+
+// out[c] = sum over rows of G[row][c] (one workgroup per column, fixed summation order).
+__global__ void __launch_bounds__(kEw) colsum_kernel(const float* __restrict__ G, int rows, int cols,
+                                                     float* __restrict__ out) {
+  __shared__ float red[kEw / 64];
+  int c = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) s += G[(long long)r * cols + c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < kEw / 64; ++w) t += red[w];
+    out[c] = t;
+  }
+}
+
+__global__ void adam_kernel(float* __restrict__ theta, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, long long n, float lr, float b1, float b2, float eps, float c1,
+                            float c2, uint16_t* __restrict__ theta_bf) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float t = theta[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  theta[i] = t;
+  theta_bf[i] = f2bf(t);
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ in, uint16_t* __restrict__ out, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = f2bf(in[i]);
+}
+
+inline unsigned ew_blocks(long long n) { return (unsigned)((n + kEw - 1) / kEw); }
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, count * sizeof(T) < 256 ? 256 : count * sizeof(T));
+}
+
+}  // namespace
+
+hipError_t mfma_gemm(const uint16_t* A, long long sam, long long sak, const uint16_t* B, long long sbk,
+                     long long sbn, float* C, int ldc, const float* bias, int act, uint16_t* out_bf16, int M, int N,
+                     int K, hipStream_t stream) {
+  dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32));
+  hipLaunchKernelGGL(gemm_kernel, grid, dim3(64), 0, stream, A, sam, sak, B, sbk, sbn, C, ldc, bias, act, out_bf16, M,
+                     N, K);
+  return hipGetLastError();
+}
+
+#define SG_TRY(expr)                         \
+  do {                                       \
+    hipError_t e_ = (expr);                  \
+    if (e_ != hipSuccess) return e_;         \
+  } while (0)
+
+hipError_t surrogate_alloc(Surrogate& s, int batch) {
+  surrogate_free(s);
+  const size_t n = MlpLayout::total;
+  SG_TRY(dalloc(&s.theta, n));
+  SG_TRY(dalloc(&s.theta_bf, n));
+  SG_TRY(dalloc(&s.adam_m, n));
+  SG_TRY(dalloc(&s.adam_v, n));
+  SG_TRY(dalloc(&s.grad, n));
+  SG_TRY(dalloc(&s.X, (size_t)batch * kMlpFeat));
+  for (int l = 0; l < 3; ++l) {
+    SG_TRY(dalloc(&s.Z[l], (size_t)batch * kMlpHidden));
+    SG_TRY(dalloc(&s.A[l], (size_t)batch * kMlpHidden));
+  }
+  SG_TRY(dalloc(&s.Y, (size_t)batch * kMlpOut));
+  SG_TRY(dalloc(&s.T, (size_t)batch * kMlpOut));
+  SG_TRY(dalloc(&s.dY, (size_t)batch * kMlpOut));
+  SG_TRY(dalloc(&s.dA, (size_t)batch * kMlpHidden));
+  SG_TRY(dalloc(&s.dZf, (size_t)batch * kMlpHidden));
+  SG_TRY(dalloc(&s.dZ, (size_t)batch * kMlpHidden));
+  SG_TRY(dalloc(&s.loss, (size_t)(batch * kMlpOut) / kEw + 2));
+  s.batch_capacity = batch;
+  return hipSuccess;
+}
+
+void surrogate_free(Surrogate& s) {
+  void* ptrs[] = {s.theta, s.theta_bf, s.adam_m, s.adam_v, s.grad, s.X, s.Z[0], s.Z[1], s.Z[2], s.A[0], s.A[1], s.A[2],
+                  s.Y,     s.T,        s.dY,     s.dA,     s.dZf,  s.dZ, s.loss};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  s = Surrogate();
+}
+
+hipError_t surrogate_init(Surrogate& s, uint64_t seed, hipStream_t stream) {
+  std::vector<float> h(MlpLayout::total, 0.f);
+  uint64_t state = seed;
+  auto uniform = [&]() {
+    state = splitmix64(state);
+    return (float)((state >> 11) * (1.0 / 9007199254740992.0));
+  };
+  auto fill = [&](size_t off, int fan_in, int fan_out, float gain) {
+    float a = gain * std::sqrt(6.0f / (float)(fan_in + fan_out));
+    for (size_t i = 0; i < (size_t)fan_in * fan_out; ++i) h[off + i] = (2.f * uniform() - 1.f) * a;
+  };
+  fill(MlpLayout::W1, kMlpFeat, kMlpHidden, 1.f);
+  fill(MlpLayout::W2, kMlpHidden, kMlpHidden, 1.f);
+  fill(MlpLayout::W3, kMlpHidden, kMlpHidden, 1.f);
+  fill(MlpLayout::W4, kMlpHidden, kMlpOut, 0.1f);
+  SG_TRY(hipMemcpyAsync(s.theta, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+  SG_TRY(hipMemsetAsync(s.adam_m, 0, h.size() * sizeof(float), stream));
+  SG_TRY(hipMemsetAsync(s.adam_v, 0, h.size() * sizeof(float), stream));
+  s.adam_step = 0;
+  s.trained = false;
+  SG_TRY(surrogate_sync_bf16(s, stream));
+  return hipStreamSynchronize(stream);
+}
+
+hipError_t surrogate_sync_bf16(Surrogate& s, hipStream_t stream) {
+  long long n = (long long)MlpLayout::total;
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks(n)), dim3(kEw), 0, stream, s.theta, s.theta_bf, n);
+  return hipGetLastError();
+}
+
+hipError_t surrogate_encode_rows(Surrogate& s, const float* inputs_dev, int rows, hipStream_t stream) {
+  hipLaunchKernelGGL(encode_rows_kernel, dim3(ew_blocks(rows)), dim3(kEw), 0, stream, inputs_dev, s.X, rows);
+  return hipGetLastError();
+}
+
+hipError_t surrogate_forward(Surrogate& s, int rows, hipStream_t stream) {
+  const uint16_t* W[4] = {s.theta_bf + MlpLayout::W1, s.theta_bf + MlpLayout::W2, s.theta_bf + MlpLayout::W3,
+                          s.theta_bf + MlpLayout::W4};
+  const float* b[4] = {s.theta + MlpLayout::b1, s.theta + MlpLayout::b2, s.theta + MlpLayout::b3,
+                       s.theta + MlpLayout::b4};
+  // hidden layers: Z = A_prev W + b, A = SiLU(Z)
+  SG_TRY(mfma_gemm(s.X, kMlpFeat, 1, W[0], kMlpHidden, 1, s.Z[0], kMlpHidden, b[0], 1, s.A[0], rows, kMlpHidden,
+                   kMlpFeat, stream));
+  SG_TRY(mfma_gemm(s.A[0], kMlpHidden, 1, W[1], kMlpHidden, 1, s.Z[1], kMlpHidden, b[1], 1, s.A[1], rows, kMlpHidden,
+                   kMlpHidden, stream));
+  SG_TRY(mfma_gemm(s.A[1], kMlpHidden, 1, W[2], kMlpHidden, 1, s.Z[2], kMlpHidden, b[2], 1, s.A[2], rows, kMlpHidden,
+                   kMlpHidden, stream));
+  SG_TRY(mfma_gemm(s.A[2], kMlpHidden, 1, W[3], kMlpOut, 1, s.Y, kMlpOut, b[3], 0, nullptr, rows, kMlpOut, kMlpHidden,
+                   stream));
+  return hipSuccess;
+}
+
+hipError_t surrogate_backward(Surrogate& s, int rows, hipStream_t stream) {
+  const long long nout = (long long)rows * kMlpOut, nh = (long long)rows * kMlpHidden;
+  // dY = mask (Y - T) / rows; dYf reuses dZf's storage (rows x 16 <= rows x 128)
+  float* dYf = s.dZf;
+  hipLaunchKernelGGL(loss_kernel, dim3(ew_blocks(nout)), dim3(kEw), 0, stream, s.Y, s.T, rows, dYf, s.dY, s.loss);
+  SG_TRY(hipGetLastError());
+  // layer 4: gW4 = A3^T dY, gb4 = colsum(dY), dA3 = dY W4^T
+  SG_TRY(mfma_gemm(s.A[2], 1, kMlpHidden, s.dY, kMlpOut, 1, s.grad + MlpLayout::W4, kMlpOut, nullptr, 0, nullptr,
+                   kMlpHidden, kMlpOut, rows, stream));
+  hipLaunchKernelGGL(colsum_kernel, dim3(kMlpOut), dim3(kEw), 0, stream, dYf, rows, kMlpOut, s.grad + MlpLayout::b4);
+  SG_TRY(hipGetLastError());
+  SG_TRY(mfma_gemm(s.dY, kMlpOut, 1, s.theta_bf + MlpLayout::W4, 1, kMlpOut, s.dA, kMlpHidden, nullptr, 0, nullptr,
+                   rows, kMlpHidden, kMlpOut, stream));
+  // hidden layers 3, 2, 1
+  const size_t Woff[3] = {MlpLayout::W1, MlpLayout::W2, MlpLayout::W3};
+  const size_t boff[3] = {MlpLayout::b1, MlpLayout::b2, MlpLayout::b3};
+  for (int l = 2; l >= 0; --l) {
+    hipLaunchKernelGGL(dsilu_kernel, dim3(ew_blocks(nh)), dim3(kEw), 0, stream, s.dA, s.Z[l], nh, s.dZf, s.dZ);
+    SG_TRY(hipGetLastError());
+    const uint16_t* prev = l == 0 ? s.X : s.A[l - 1];
+    const int kin = l == 0 ? kMlpFeat : kMlpHidden;
+    SG_TRY(mfma_gemm(prev, 1, kin, s.dZ, kMlpHidden, 1, s.grad + Woff[l], kMlpHidden, nullptr, 0, nullptr, kin,
+                     kMlpHidden, rows, stream));
+    hipLaunchKernelGGL(colsum_kernel, dim3(kMlpHidden), dim3(kEw), 0, stream, s.dZf, rows, kMlpHidden,
+                       s.grad + boff[l]);
+    SG_TRY(hipGetLastError());
+    if (l > 0)
+      SG_TRY(mfma_gemm(s.dZ, kMlpHidden, 1, s.theta_bf + Woff[l], 1, kMlpHidden, s.dA, kMlpHidden, nullptr, 0,
+                       nullptr, rows, kMlpHidden, kMlpHidden, stream));
+  }
+  return hipSuccess;
+}
+
+hipError_t surrogate_adam(Surrogate& s, float lr, hipStream_t stream) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  s.adam_step += 1;
+  float c1 = 1.f - std::pow(b1, (float)s.adam_step), c2 = 1.f - std::pow(b2, (float)s.adam_step);
+  long long n = (long long)MlpLayout::total;
+  hipLaunchKernelGGL(adam_kernel, dim3(ew_blocks(n)), dim3(kEw), 0, stream, s.theta, s.grad, s.adam_m, s.adam_v, n, lr,
+                     b1, b2, eps, c1, c2, s.theta_bf);
+  return hipGetLastError();
+}
+
+hipError_t surrogate_gather_batch(Surrogate& s, const DevParams& P, const AtmosphereCode* codes_dev, int natm,
+                                  const float* targets_dev, uint64_t seed, int step, int rows, hipStream_t stream) {
+  hipLaunchKernelGGL(gather_kernel, dim3(ew_blocks(rows)), dim3(kEw), 0, stream, P, codes_dev, natm, targets_dev, seed,
+                     step, rows, s.X, s.T);
+  return hipGetLastError();
+}
+
+hipError_t surrogate_make_targets(const float4* p4, const float4* p3, const float4* p5, long long entries,
+                                  float* targets_dev, hipStream_t stream) {
+  hipLaunchKernelGGL(targets_kernel, dim3(ew_blocks(entries)), dim3(kEw), 0, stream, p4, p3, p5, entries, targets_dev);
+  return hipGetLastError();
+}
+
+hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* p4,
+                                    float4* p3, float4* p5, hipStream_t stream) {
+  const long long entries = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
+  for (long long first = 0; first < entries; first += s.batch_capacity) {
+    int rows = (int)std::min<long long>(s.batch_capacity, entries - first);
+    hipLaunchKernelGGL(encode_entries_kernel, dim3(ew_blocks(rows)), dim3(kEw), 0, stream, P, code, first, rows, s.X);
+    SG_TRY(hipGetLastError());
+    SG_TRY(surrogate_forward(s, rows, stream));
+    hipLaunchKernelGGL(decode_kernel, dim3(ew_blocks(rows)), dim3(kEw), 0, stream, s.Y, first, rows, p4, p3, p5);
+    SG_TRY(hipGetLastError());
+  }
+  return hipSuccess;
+}
+
+AtmosphereCode atmosphere_code(const double* rayleigh_base, double rayleigh_scale, double mie_base, double mie_scale,
+                               double mie_g, double albedo) {
+  auto unit = [](double v, double lo, double hi) { return (float)std::fmin(std::fmax((v - lo) / (hi - lo), 0.0), 1.0); };
+  AtmosphereCode c;
+  c.v[0] = unit(rayleigh_scale, 6000.0, 10000.0);
+  c.v[1] = unit(rayleigh_base[0] / 5.8e-6, 0.5, 2.0);
+  c.v[2] = unit(mie_scale, 800.0, 2000.0);
+  c.v[3] = unit(mie_base, 5e-6, 5e-5);
+  c.v[4] = unit(mie_g, 0.6, 0.85);
+  c.v[5] = unit(albedo, 0.1, 0.5);
+  return c;
+}
+
+}  // namespace sfsim

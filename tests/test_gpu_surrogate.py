@@ -1,0 +1,253 @@
+"""GPU checks of the learned stand-in for scattering orders >= 2 (no counterpart in the reference).
+
+* forward pass (bf16 matrix-core products, FP32 accumulation) against the FP64 reference network
+  oracle/mlp.py, both exact and with the GPU's bf16 roundings emulated;
+* loss and gradient (every parameter block; all three operand layouts of the matrix-core product:
+  A B, A^T B and A B^T) against oracle/mlp.py;
+* the order sums P3, P4, P5 that the network learns, against the oracle's own iteration;
+* training: the loss falls, and two runs from one seed give identical weights;
+* the seeded 5-order build: better than the same step without the network, falls back to the plain
+  iteration (bit-identical result) when the residual exceeds the tolerance, and refuses an untrained
+  network.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp
+from tests.parity import TOL_SCATTER, assert_close, oracle_params, rel_errors
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def atm():
+    from sfsim_amd import atmosphere
+    return atmosphere
+
+
+@pytest.fixture(scope="module")
+def gpu(atm):
+    ctx = atm.Atmosphere(0)
+    yield ctx
+    ctx.close()
+
+
+def small5(atm, **kw):
+    base = dict(planet={"height": 100000.0}, transmittance_shape=(10, 17), surface_radiance_shape=(6, 11),
+                scatter_shape=(6, 11, 7, 4), ray_steps=16, sphere_steps=10, iterations=5)
+    base.update(kw)
+    return atm.make_params(**base)
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest, ties to even), returned as float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def xavier_theta(rng):
+    parts = []
+    for s in mlp.SHAPES:
+        if len(s) == 2:
+            a = np.sqrt(6.0 / (s[0] + s[1]))
+            parts.append(rng.uniform(-a, a, s))
+        else:
+            parts.append(rng.normal(0.0, 0.1, s))
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
+def emulated_forward(theta, x):
+    """The GPU's forward pass with its bf16 roundings (features, weights, activations), FP64 sums."""
+    W1, b1, W2, b2, W3, b3, W4, b4 = mlp.unflatten(theta)
+    a = bf16(mlp.encode(x).astype(np.float32))
+    for W, b in ((W1, b1), (W2, b2), (W3, b3)):
+        a = bf16(mlp.silu(a @ bf16(W) + b).astype(np.float32))
+    return a @ bf16(W4) + b4
+
+
+@pytest.fixture(scope="module")
+def net(atm, gpu):
+    rng = np.random.default_rng(20261202)
+    theta = xavier_theta(rng)
+    sg = atm.Surrogate(gpu, seed=1, batch=256)
+    sg.set_weights(theta, trained=False)
+    x = rng.uniform(0, 1, (200, 10)).astype(np.float32)
+    return sg, theta.astype(np.float64), x
+
+
+def test_parameter_count(atm):
+    assert atm.surrogate_size() == mlp.n_params()
+
+
+def test_forward_matches_fp64_network(net):
+    sg, theta, x = net
+    y, _, _ = sg.eval(x)
+    ref = mlp.forward(theta, mlp.encode(x))[0]
+    emu = emulated_forward(theta, x)
+    scale = np.max(np.abs(ref))
+    assert scale > 0.1
+    # bf16 operands: about 3 significant digits; a transposed operand or a wrong accumulator layout
+    # gives errors of the order of the outputs themselves
+    assert np.max(np.abs(y - emu)) <= 1e-2 * scale
+    assert np.max(np.abs(y - ref)) <= 5e-2 * scale
+    assert np.median(np.abs(y - emu)) <= 1e-3 * scale
+
+
+def test_forward_single_row_and_weight_roundtrip(atm, net):
+    sg, theta, x = net
+    y_all, _, _ = sg.eval(x)
+    y_one, _, _ = sg.eval(x[17:18])
+    assert np.array_equal(y_one[0], y_all[17])
+    assert np.array_equal(sg.weights(), theta.astype(np.float32))
+
+
+def test_loss_and_gradient_match_fp64_network(net):
+    sg, theta, x = net
+    rng = np.random.default_rng(7)
+    T = np.zeros((x.shape[0], 16), np.float32)
+    T[:, :9] = rng.normal(0.0, 1.0, (x.shape[0], 9))
+    y, g, loss = sg.eval(x, T)
+    mask = np.zeros((x.shape[0], 16))
+    mask[:, :9] = 1.0
+    ref_loss, ref_g = mlp.loss_and_grad(theta, mlp.encode(x), T.astype(np.float64), mask)
+    assert loss == pytest.approx(ref_loss, rel=2e-2)
+    offsets = np.cumsum([0] + [int(np.prod(s)) for s in mlp.SHAPES])
+    names = ["W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4"]
+    for b, name in enumerate(names):
+        gg, rr = g[offsets[b]:offsets[b + 1]].astype(np.float64), ref_g[offsets[b]:offsets[b + 1]]
+        rel = np.linalg.norm(gg - rr) / np.linalg.norm(rr)
+        cos = gg @ rr / (np.linalg.norm(gg) * np.linalg.norm(rr))
+        print(f"{name}: relative error {rel:.2e}, cosine {cos:.6f}")
+        assert rel <= 3e-2 and cos >= 0.999, name
+    # outputs 9..15 are masked: their weight and bias gradients are exactly zero
+    W4g = g[offsets[6]:offsets[7]].reshape(128, 16)
+    assert not W4g[:, 9:].any() and not g[offsets[7] + 9:offsets[8]].any()
+
+
+def _oracle_order_sums(oracle, OP):
+    R1 = oracle.stage_single_scatter(OP, 0)
+    M1 = oracle.stage_single_scatter(OP, 1)
+    dS, dE = R1, oracle.stage_surface_radiance_base(OP)
+    sums, total = [], 0.0
+    for k in range(1, 6):
+        first = 1 if k == 1 else 0
+        mie = M1 if first else None
+        J = oracle.stage_point_scatter(OP, first, dS, mie, dE)
+        dE = oracle.stage_surface_radiance(OP, first, dS, mie)
+        dS = oracle.stage_ray_scatter(OP, J)
+        total = total + dS
+        sums.append(total)
+    return sums[2], sums[3], sums[4]
+
+
+def test_order_sums_match_oracle(atm, gpu, oracle):
+    P = small5(atm)
+    p3, p4, p5 = atm.order_sums(gpu, P)
+    r3, r4, r5 = _oracle_order_sums(oracle, oracle_params(oracle, P))
+    for name, got, ref in (("P3", p3, r3), ("P4", p4, r4), ("P5", p5, r5)):
+        assert_close(name, got, ref, TOL_SCATTER)
+    assert (p3 >= 0).all() and (p4 >= p3).all() and (p5 >= p4).all()
+
+
+@pytest.fixture(scope="module")
+def trained(atm, gpu):
+    """A network trained on the test atmosphere and two neighbours (small grid)."""
+    P = small5(atm)
+    rng = np.random.default_rng(3)
+    family = [P] + [atm.random_atmosphere(rng, P) for _ in range(2)]
+    sg = atm.Surrogate(gpu, seed=11, batch=1024)
+    losses = sg.train(family, steps=1500, lr=3e-3, seed=5)
+    return sg, P, family, losses
+
+
+def test_training_reduces_loss(trained):
+    sg, P, family, losses = trained
+    assert np.all(np.isfinite(losses))
+    first, last = losses[:10].mean(), losses[-50:].mean()
+    print(f"loss {first:.4g} -> {last:.4g}")
+    assert last < 0.02 * first
+
+
+def test_training_is_deterministic(atm, gpu, trained):
+    sg, P, family, _ = trained
+    w = sg.weights()
+    other = atm.Atmosphere(0)
+    try:
+        sg2 = atm.Surrogate(other, seed=11, batch=1024)
+        sg2.train(family, steps=1500, lr=3e-3, seed=5)
+        assert np.array_equal(sg2.weights(), w)
+    finally:
+        other.close()
+
+
+def test_prediction_tracks_order_sums(atm, gpu, trained):
+    sg, P, family, _ = trained
+    p3, p4, p5 = atm.order_sums(gpu, P)
+    q3, q4, q5 = sg.predict(P)
+    for name, got, ref in (("P3", q3, p3), ("P4", q4, p4), ("P5", q5, p5)):
+        err = rel_errors(got, ref)
+        print(f"{name}: median {np.median(err):.3e}, max {err.max():.3e}")
+        assert np.median(err) < 0.1
+
+
+def test_seeded_build_beats_the_step_without_network(atm, gpu, trained):
+    sg, P, family, _ = trained
+    gpu.build(P)
+    plain = [gpu.fetch(P, i) for i in range(4)]
+    r, fell_back = atm.build_seeded(gpu, P, tolerance=1.0)
+    assert not fell_back and np.isfinite(r) and r >= 0
+    seeded = [gpu.fetch(P, i) for i in range(4)]
+    err_seeded = max(rel_errors(seeded[i], plain[i]).max() for i in (1, 2))
+    assert np.array_equal(seeded[0], plain[0]) and np.array_equal(seeded[3], plain[3])
+    # without the network: the same single step from P = 0 (two scattering orders instead of five)
+    w = sg.weights()
+    zero = np.zeros_like(w)
+    zero[-16:-7] = 10.0 + np.log(1e-8)
+    sg.set_weights(zero, trained=True)
+    try:
+        atm.build_seeded(gpu, P, tolerance=np.inf)
+        trunc = [gpu.fetch(P, i) for i in range(4)]
+    finally:
+        sg.set_weights(w, trained=True)
+    err_trunc = max(rel_errors(trunc[i], plain[i]).max() for i in (1, 2))
+    print(f"seeded {err_seeded:.3e} (residual {r:.3e}), without network {err_trunc:.3e}")
+    assert err_seeded < 0.2 * err_trunc
+
+
+def test_seeded_build_falls_back_to_plain(atm, gpu, trained):
+    sg, P, family, _ = trained
+    gpu.build(P)
+    plain = [gpu.fetch(P, i) for i in range(4)]
+    r, fell_back = atm.build_seeded(gpu, P, tolerance=0.0)
+    assert r > 0 and fell_back
+    for i in range(4):
+        assert np.array_equal(gpu.fetch(P, i), plain[i])
+
+
+def test_seeded_build_refuses_untrained_network(atm):
+    ctx = atm.Atmosphere(0)
+    try:
+        atm.Surrogate(ctx, seed=1, batch=256)
+        with pytest.raises(atm.AtmoError):
+            atm.build_seeded(ctx, small5(atm))
+        with pytest.raises(atm.AtmoError):
+            atm.build_seeded(ctx, small5(atm, iterations=3))
+    finally:
+        ctx.close()
+
+
+def test_surrogate_argument_errors(atm, gpu):
+    ctx = atm.Atmosphere(0)
+    try:
+        with pytest.raises(atm.AtmoError):
+            atm.Surrogate(ctx, seed=1, batch=100)  # not a multiple of 64
+        sg = atm.Surrogate(ctx, seed=1, batch=64)
+        with pytest.raises(atm.AtmoError):
+            sg.eval(np.zeros((65, 10), np.float32))  # more rows than the batch capacity
+        bad = np.zeros(atm.surrogate_size(), np.float32)
+        bad[3] = np.nan
+        with pytest.raises(atm.AtmoError):
+            sg.set_weights(bad)
+    finally:
+        ctx.close()
