@@ -42,24 +42,29 @@ __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
 // One wavefront computes a 32 x 32 tile of C with v_mfma_f32_32x32x8bf16_1k.  Operand layout of that
 // instruction: lane l supplies A(l % 32, 4 (l / 32) + i) and B(4 (l / 32) + i, l % 32), i = 0..3;
 // accumulator element e of lane l is C(8 (e / 4) + 4 (l / 32) + e % 4, l % 32).
+// Split K: workgroup z covers k in [z kc, min(K, (z + 1) kc)) and writes its partial product to
+// C + z * split_stride (the caller sums the partials in a fixed order).
 __global__ void __launch_bounds__(64) gemm_kernel(const uint16_t* __restrict__ A, long long sam, long long sak,
                                                   const uint16_t* __restrict__ B, long long sbk, long long sbn,
                                                   float* __restrict__ C, int ldc, const float* __restrict__ bias,
-                                                  int act, uint16_t* __restrict__ outb, int M, int N, int K) {
+                                                  int act, uint16_t* __restrict__ outb, int M, int N, int K, int kc,
+                                                  long long split_stride) {
   const int lane = threadIdx.x;
   const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
   const int r = lane & 31, kq = (lane >> 5) * 4;
   const int am = m0 + r, bn = n0 + r;
   const bool arow = am < M, bcol = bn < N;
+  const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
+  C += blockIdx.z * split_stride;
   float16_t acc;
 #pragma unroll
   for (int e = 0; e < 16; ++e) acc[e] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += 8) {
+  for (int k0 = kbeg; k0 < kend; k0 += 8) {
     short4_t va, vb;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       int k = k0 + kq + i;
-      bool kin = k < K;
+      bool kin = k < kend;
       va[i] = (short)((arow && kin) ? A[am * sam + k * sak] : 0);
       vb[i] = (short)((bcol && kin) ? B[k * sbk + bn * sbn] : 0);
     }
@@ -78,22 +83,23 @@ __global__ void __launch_bounds__(64) gemm_kernel(const uint16_t* __restrict__ A
   }
 }
 
-// Fourier features of one input row: x, then sin / cos (2^b pi x) for b = 0..7, per input; zero pad.
-__device__ inline void encode_row(const float* x, uint16_t* out) {
-  int f = 0;
-  for (int i = 0; i < kMlpIn; ++i) {
-    float v = x[i];
-    out[f++] = f2bf(v);
-    float a = 3.14159265358979f * v;
-    for (int b = 0; b < 8; ++b) {
-      float sn, cs;
-      sincosf(a, &sn, &cs);
-      out[f++] = f2bf(sn);
-      out[f++] = f2bf(cs);
-      a *= 2.0f;
-    }
+// The 17 Fourier features of one input x: x, then sin / cos (2^b pi x) for b = 0..7 (bf16).
+__device__ inline void encode_value(float v, uint16_t* out) {
+  out[0] = f2bf(v);
+  float a = 3.14159265358979f * v;
+  for (int b = 0; b < 8; ++b) {
+    float sn, cs;
+    sincosf(a, &sn, &cs);
+    out[1 + 2 * b] = f2bf(sn);
+    out[2 + 2 * b] = f2bf(cs);
+    a *= 2.0f;  // exact
   }
-  for (; f < kMlpFeat; ++f) out[f] = 0;
+}
+
+// Features of one input row: the 17 features of each of the 10 inputs, zero padded to 192.
+__device__ inline void encode_row(const float* x, uint16_t* out) {
+  for (int i = 0; i < kMlpIn; ++i) encode_value(x[i], out + kMlpBands * i);
+  for (int f = kMlpBands * kMlpIn; f < kMlpFeat; ++f) out[f] = 0;
 }
 
 __global__ void encode_rows_kernel(const float* __restrict__ in, uint16_t* __restrict__ X, int rows) {
@@ -131,28 +137,6 @@ __global__ void gather_kernel(const DevParams P, const AtmosphereCode* __restric
   encode_row(x, X + (long long)r * kMlpFeat);
   const float* t = targets + ((long long)atm * entries + entry) * kMlpUsed;
   for (int o = 0; o < kMlpOut; ++o) T[(long long)r * kMlpOut + o] = o < kMlpUsed ? t[o] : 0.f;
-}
-
-__global__ void encode_entries_kernel(const DevParams P, AtmosphereCode code, long long first, int rows,
-                                      uint16_t* __restrict__ X) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  float x[kMlpIn];
-  entry_coords(P, first + r, x);
-  for (int i = 0; i < 6; ++i) x[4 + i] = code.v[i];
-  encode_row(x, X + (long long)r * kMlpFeat);
-}
-
-__global__ void decode_kernel(const float* __restrict__ Y, long long first, int rows, float4* __restrict__ p4,
-                              float4* __restrict__ p3, float4* __restrict__ p5) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  const float* y = Y + (long long)r * kMlpOut;
-  float v[kMlpUsed];
-  for (int o = 0; o < kMlpUsed; ++o) v[o] = fmaxf(expf(y[o] - kMlpShift) - kMlpEps, 0.f);
-  p4[first + r] = make_float4(v[0], v[1], v[2], 0.f);
-  p3[first + r] = make_float4(v[3], v[4], v[5], 0.f);
-  p5[first + r] = make_float4(v[6], v[7], v[8], 0.f);
 }
 
 __global__ void targets_kernel(const float4* __restrict__ p4, const float4* __restrict__ p3,
@@ -235,6 +219,167 @@ __global__ void to_bf16_kernel(const float* __restrict__ in, uint16_t* __restric
   if (i < n) out[i] = f2bf(in[i]);
 }
 
+// ---- fused inference over a scatter grid ---------------------------------------------------------
+// Layer 1 is linear in the features and the features of an entry split into those of its 4 grid
+// coordinates and those of the 6 atmosphere inputs, which one prediction holds fixed.  So
+//   Z1(entry) = b1 + W1_atm^T f(atm) + sum_c W1_c^T f(x_c(entry)),
+// and the terms are tabulated once per prediction: row 0 = b1 + W1_atm^T f(atm), then one row of 128
+// per value of each grid coordinate (heights, elevations, sun elevations, headings).
+__global__ void layer1_tables_kernel(const DevParams P, AtmosphereCode code, const uint16_t* __restrict__ wbf,
+                                     const float* __restrict__ theta, float* __restrict__ ztab) {
+  const int j = threadIdx.x;  // hidden unit (blockDim.x == kMlpHidden)
+  int row = blockIdx.x;
+  uint16_t f[kMlpBands];
+  const uint16_t* W1 = wbf + MlpLayout::W1;
+  float acc;
+  if (row == 0) {
+    acc = theta[MlpLayout::b1 + j];
+    for (int i = 0; i < 6; ++i) {
+      encode_value(code.v[i], f);
+      for (int q = 0; q < kMlpBands; ++q)
+        acc += bf2f(W1[(size_t)(kMlpBands * (4 + i) + q) * kMlpHidden + j]) * bf2f(f[q]);
+    }
+  } else {
+    int v = row - 1, c = 0;
+    while (v >= P.nS[c]) v -= P.nS[c++];
+    encode_value((float)v / (float)(P.nS[c] - 1), f);
+    acc = 0.f;
+    for (int q = 0; q < kMlpBands; ++q) acc += bf2f(W1[(size_t)(kMlpBands * c + q) * kMlpHidden + j]) * bf2f(f[q]);
+  }
+  ztab[(size_t)blockIdx.x * kMlpHidden + j] = acc;
+}
+
+constexpr int kFusedWaves = 4;
+constexpr int kLdsRow = kMlpHidden + 4;  // padded row (bf16) of the transposed weights in LDS
+constexpr int kFusedLds = (2 * kMlpHidden + 32) * kLdsRow * 2;
+
+__device__ __forceinline__ short4_t silu_bf16x4(float a, float b, float c, float d) {
+  short4_t r;
+  r[0] = (short)f2bf(silu(a));
+  r[1] = (short)f2bf(silu(b));
+  r[2] = (short)f2bf(silu(c));
+  r[3] = (short)f2bf(silu(d));
+  return r;
+}
+
+// One wavefront evaluates 32 entries at a time with the hidden units as the row dimension of every
+// product (Z^T = W^T A^T): the accumulator of v_mfma_f32_32x32x8bf16_1k leaves lane l with hidden units
+// 8 g + 4 (l / 32) + i (g = 0..3, i = 0..3) of entry l % 32, which is exactly the lane's B operand of
+// the next layer's K-steps 8 g .. 8 g + 7, so activations never leave the registers.  W2^T, W3^T and
+// W4^T (padded to 32 rows) are staged once per workgroup in LDS; workgroups loop over the tiles.
+__global__ void __launch_bounds__(64 * kFusedWaves) fused_predict_kernel(
+    const DevParams P, const uint16_t* __restrict__ wbf, const float* __restrict__ theta,
+    const float* __restrict__ ztab, long long entries, float4* __restrict__ p4, float4* __restrict__ p3,
+    float4* __restrict__ p5) {
+  __shared__ __align__(16) uint16_t lds[kFusedLds / 2];
+  uint16_t* w2t = lds;
+  uint16_t* w3t = w2t + kMlpHidden * kLdsRow;
+  uint16_t* w4t = w3t + kMlpHidden * kLdsRow;
+  for (int i = threadIdx.x; i < kMlpHidden * kMlpHidden; i += blockDim.x) {
+    const int k = i / kMlpHidden, n = i % kMlpHidden;
+    w2t[n * kLdsRow + k] = wbf[MlpLayout::W2 + i];
+    w3t[n * kLdsRow + k] = wbf[MlpLayout::W3 + i];
+  }
+  for (int i = threadIdx.x; i < 32 * kMlpHidden; i += blockDim.x) {
+    const int n = i / kMlpHidden, k = i % kMlpHidden;
+    w4t[n * kLdsRow + k] = n < kMlpOut ? wbf[MlpLayout::W4 + k * kMlpOut + n] : (uint16_t)0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 31, h = lane >> 5;
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const float* b2 = theta + MlpLayout::b2;
+  const float* b3 = theta + MlpLayout::b3;
+  const float* b4 = theta + MlpLayout::b4;
+  const long long tiles = (entries + 31) / 32;
+  for (long long t = (long long)blockIdx.x * kFusedWaves + wave; t < tiles; t += (long long)gridDim.x * kFusedWaves) {
+    const long long e = t * 32 + r;
+    const long long ec = e < entries ? e : entries - 1;
+    const int ia = (int)(ec % NA);
+    long long q = ec / NA;
+    const int is = (int)(q % NL);
+    q /= NL;
+    const int ie = (int)(q % NE);
+    const int ih = (int)(q / NE);
+    const float4* z0 = (const float4*)ztab;
+    const float4* zh = (const float4*)(ztab + (size_t)(1 + ih) * kMlpHidden);
+    const float4* ze = (const float4*)(ztab + (size_t)(1 + NH + ie) * kMlpHidden);
+    const float4* zs = (const float4*)(ztab + (size_t)(1 + NH + NE + is) * kMlpHidden);
+    const float4* za = (const float4*)(ztab + (size_t)(1 + NH + NE + NL + ia) * kMlpHidden);
+    short4_t b[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int j4 = 2 * s + h;  // hidden units 8 s + 4 h .. + 3
+      const float4 a0 = z0[j4], a1 = zh[j4], a2 = ze[j4], a3 = zs[j4], a4 = za[j4];
+      b[s] = silu_bf16x4(a0.x + a1.x + a2.x + a3.x + a4.x, a0.y + a1.y + a2.y + a3.y + a4.y,
+                         a0.z + a1.z + a2.z + a3.z + a4.z, a0.w + a1.w + a2.w + a3.w + a4.w);
+    }
+#pragma unroll 1
+    for (int layer = 0; layer < 2; ++layer) {
+      const uint16_t* wt = layer ? w3t : w2t;
+      const float* bias = layer ? b3 : b2;
+      float16_t acc[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int x = 0; x < 16; ++x) acc[mt][x] = 0.f;
+#pragma unroll
+      for (int s = 0; s < 16; ++s)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          const short4_t a = *(const short4_t*)(wt + (32 * mt + r) * kLdsRow + 8 * s + 4 * h);
+          acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[s], acc[mt], 0, 0, 0);
+        }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const float4 bb = *(const float4*)(bias + 32 * mt + 8 * g + 4 * h);
+          b[4 * mt + g] = silu_bf16x4(acc[mt][4 * g] + bb.x, acc[mt][4 * g + 1] + bb.y, acc[mt][4 * g + 2] + bb.z,
+                                      acc[mt][4 * g + 3] + bb.w);
+        }
+    }
+    float16_t y;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) y[x] = 0.f;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const short4_t a = *(const short4_t*)(w4t + r * kLdsRow + 8 * s + 4 * h);
+      y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[s], y, 0, 0, 0);
+    }
+    // lane (r, h) holds outputs 4 h + i (y[i]) and 8 + 4 h + i (y[4 + i]) of entry r
+    float o[kMlpUsed];
+    const float u0 = __shfl(y[0], r + 32, 64), u1 = __shfl(y[1], r + 32, 64), u2 = __shfl(y[2], r + 32, 64),
+                u3 = __shfl(y[3], r + 32, 64);
+    o[0] = y[0];
+    o[1] = y[1];
+    o[2] = y[2];
+    o[3] = y[3];
+    o[4] = u0;
+    o[5] = u1;
+    o[6] = u2;
+    o[7] = u3;
+    o[8] = y[4];
+    if (h == 0 && e < entries) {
+      float v[kMlpUsed];
+#pragma unroll
+      for (int k = 0; k < kMlpUsed; ++k) v[k] = fmaxf(expf(o[k] + b4[k] - kMlpShift) - kMlpEps, 0.f);
+      p4[e] = make_float4(v[0], v[1], v[2], 0.f);
+      p3[e] = make_float4(v[3], v[4], v[5], 0.f);
+      p5[e] = make_float4(v[6], v[7], v[8], 0.f);
+    }
+  }
+}
+
+// C[i] = sum over z of partial[z][i], z in order.
+__global__ void split_sum_kernel(const float* __restrict__ partial, int splits, long long n, float* __restrict__ C) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += partial[z * n + i];
+  C[i] = s;
+}
+
 inline unsigned ew_blocks(long long n) { return (unsigned)((n + kEw - 1) / kEw); }
 
 template <typename T>
@@ -249,7 +394,26 @@ hipError_t mfma_gemm(const uint16_t* A, long long sam, long long sak, const uint
                      int K, hipStream_t stream) {
   dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32));
   hipLaunchKernelGGL(gemm_kernel, grid, dim3(64), 0, stream, A, sam, sak, B, sbk, sbn, C, ldc, bias, act, out_bf16, M,
-                     N, K);
+                     N, K, K, 0LL);
+  return hipGetLastError();
+}
+
+// Weight gradients (M x N small, K = rows large): K split over kMaxSplits workgroups per tile, partial
+// products in `partial` (kMaxSplits x M x N floats), then summed in split order: deterministic.
+hipError_t mfma_gemm_split_k(const uint16_t* A, long long sam, long long sak, const uint16_t* B, long long sbk,
+                             long long sbn, float* C, int M, int N, int K, float* partial, hipStream_t stream) {
+  int kc = (K + kMaxSplits - 1) / kMaxSplits;
+  kc = (kc + 7) / 8 * 8;
+  if (kc < 64) kc = 64;
+  const int splits = (K + kc - 1) / kc;
+  dim3 grid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32), (unsigned)splits);
+  hipLaunchKernelGGL(gemm_kernel, grid, dim3(64), 0, stream, A, sam, sak, B, sbk, sbn, partial, N, nullptr, 0,
+                     nullptr, M, N, K, kc, (long long)M * N);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long n = (long long)M * N;
+  hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)((n + kEw - 1) / kEw)), dim3(kEw), 0, stream, partial, splits, n,
+                     C);
   return hipGetLastError();
 }
 
@@ -279,13 +443,14 @@ hipError_t surrogate_alloc(Surrogate& s, int batch) {
   SG_TRY(dalloc(&s.dZf, (size_t)batch * kMlpHidden));
   SG_TRY(dalloc(&s.dZ, (size_t)batch * kMlpHidden));
   SG_TRY(dalloc(&s.loss, (size_t)(batch * kMlpOut) / kEw + 2));
+  SG_TRY(dalloc(&s.partial, (size_t)kMaxSplits * kMlpFeat * kMlpHidden));
   s.batch_capacity = batch;
   return hipSuccess;
 }
 
 void surrogate_free(Surrogate& s) {
-  void* ptrs[] = {s.theta, s.theta_bf, s.adam_m, s.adam_v, s.grad, s.X, s.Z[0], s.Z[1], s.Z[2], s.A[0], s.A[1], s.A[2],
-                  s.Y,     s.T,        s.dY,     s.dA,     s.dZf,  s.dZ, s.loss};
+  void* ptrs[] = {s.theta, s.theta_bf, s.adam_m, s.adam_v, s.grad, s.X,  s.Z[0], s.Z[1], s.Z[2], s.A[0], s.A[1],
+                  s.A[2],  s.Y,        s.T,      s.dY,     s.dA,   s.dZf, s.dZ,   s.loss, s.ztab, s.partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   s = Surrogate();
@@ -350,8 +515,8 @@ hipError_t surrogate_backward(Surrogate& s, int rows, hipStream_t stream) {
   hipLaunchKernelGGL(loss_kernel, dim3(ew_blocks(nout)), dim3(kEw), 0, stream, s.Y, s.T, rows, dYf, s.dY, s.loss);
   SG_TRY(hipGetLastError());
   // layer 4: gW4 = A3^T dY, gb4 = colsum(dY), dA3 = dY W4^T
-  SG_TRY(mfma_gemm(s.A[2], 1, kMlpHidden, s.dY, kMlpOut, 1, s.grad + MlpLayout::W4, kMlpOut, nullptr, 0, nullptr,
-                   kMlpHidden, kMlpOut, rows, stream));
+  SG_TRY(mfma_gemm_split_k(s.A[2], 1, kMlpHidden, s.dY, kMlpOut, 1, s.grad + MlpLayout::W4, kMlpHidden, kMlpOut,
+                           rows, s.partial, stream));
   hipLaunchKernelGGL(colsum_kernel, dim3(kMlpOut), dim3(kEw), 0, stream, dYf, rows, kMlpOut, s.grad + MlpLayout::b4);
   SG_TRY(hipGetLastError());
   SG_TRY(mfma_gemm(s.dY, kMlpOut, 1, s.theta_bf + MlpLayout::W4, 1, kMlpOut, s.dA, kMlpHidden, nullptr, 0, nullptr,
@@ -364,8 +529,8 @@ hipError_t surrogate_backward(Surrogate& s, int rows, hipStream_t stream) {
     SG_TRY(hipGetLastError());
     const uint16_t* prev = l == 0 ? s.X : s.A[l - 1];
     const int kin = l == 0 ? kMlpFeat : kMlpHidden;
-    SG_TRY(mfma_gemm(prev, 1, kin, s.dZ, kMlpHidden, 1, s.grad + Woff[l], kMlpHidden, nullptr, 0, nullptr, kin,
-                     kMlpHidden, rows, stream));
+    SG_TRY(mfma_gemm_split_k(prev, 1, kin, s.dZ, kMlpHidden, 1, s.grad + Woff[l], kin, kMlpHidden, rows, s.partial,
+                             stream));
     hipLaunchKernelGGL(colsum_kernel, dim3(kMlpHidden), dim3(kEw), 0, stream, s.dZf, rows, kMlpHidden,
                        s.grad + boff[l]);
     SG_TRY(hipGetLastError());
@@ -402,15 +567,23 @@ hipError_t surrogate_make_targets(const float4* p4, const float4* p3, const floa
 hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* p4,
                                     float4* p3, float4* p5, hipStream_t stream) {
   const long long entries = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
-  for (long long first = 0; first < entries; first += s.batch_capacity) {
-    int rows = (int)std::min<long long>(s.batch_capacity, entries - first);
-    hipLaunchKernelGGL(encode_entries_kernel, dim3(ew_blocks(rows)), dim3(kEw), 0, stream, P, code, first, rows, s.X);
-    SG_TRY(hipGetLastError());
-    SG_TRY(surrogate_forward(s, rows, stream));
-    hipLaunchKernelGGL(decode_kernel, dim3(ew_blocks(rows)), dim3(kEw), 0, stream, s.Y, first, rows, p4, p3, p5);
-    SG_TRY(hipGetLastError());
+  const int rows = 1 + P.nS[0] + P.nS[1] + P.nS[2] + P.nS[3];
+  if ((size_t)rows * kMlpHidden > s.ztab_capacity) {
+    if (s.ztab) SG_TRY(hipFree(s.ztab));
+    s.ztab = nullptr;
+    s.ztab_capacity = 0;
+    SG_TRY(dalloc(&s.ztab, (size_t)rows * kMlpHidden));
+    s.ztab_capacity = (size_t)rows * kMlpHidden;
   }
-  return hipSuccess;
+  hipLaunchKernelGGL(layer1_tables_kernel, dim3(rows), dim3(kMlpHidden), 0, stream, P, code, s.theta_bf, s.theta,
+                     s.ztab);
+  SG_TRY(hipGetLastError());
+  const long long tiles = (entries + 31) / 32;
+  const long long groups = (tiles + kFusedWaves - 1) / kFusedWaves;
+  const unsigned grid = (unsigned)std::min<long long>(groups, 1024);
+  hipLaunchKernelGGL(fused_predict_kernel, dim3(grid), dim3(64 * kFusedWaves), 0, stream, P, s.theta_bf,
+                     s.theta, s.ztab, entries, p4, p3, p5);
+  return hipGetLastError();
 }
 
 AtmosphereCode atmosphere_code(const double* rayleigh_base, double rayleigh_scale, double mie_base, double mie_scale,

@@ -18,12 +18,14 @@
 namespace sfsim {
 
 constexpr int kMlpIn = 10;
+constexpr int kMlpBands = 17;  // features per input: x, sin and cos of 2^b pi x for b = 0..7
 constexpr int kMlpFeat = 192;
 constexpr int kMlpHidden = 128;
 constexpr int kMlpOut = 16;
 constexpr int kMlpUsed = 9;
 constexpr float kMlpEps = 1e-8f;     // floor inside the log
 constexpr float kMlpShift = 10.0f;   // targets are log(P + eps) + shift
+constexpr int kMaxSplits = 32;       // K splits of the weight-gradient products
 
 // Offsets of the parameter blocks in the flat FP32 vector (W1 b1 W2 b2 W3 b3 W4 b4, row-major W[in][out]).
 struct MlpLayout {
@@ -66,6 +68,9 @@ struct Surrogate {
   float* dZf = nullptr;        // [B][128] FP32 gradient before the activation (bias gradients)
   uint16_t* dZ = nullptr;      // [B][128] bf16
   float* loss = nullptr;       // [1 + B / 256] partial sums
+  float* partial = nullptr;    // [kMaxSplits][192][128] split-K partial products
+  float* ztab = nullptr;       // layer-1 tables of the fused prediction
+  size_t ztab_capacity = 0;
   int adam_step = 0;
 };
 
@@ -93,7 +98,8 @@ hipError_t surrogate_gather_batch(Surrogate& s, const DevParams& P, const Atmosp
 // Targets of one atmosphere from its sums P3, P4, P5 of the plain iteration (float4 tables).
 hipError_t surrogate_make_targets(const float4* p4, const float4* p3, const float4* p5, long long entries,
                                   float* targets_dev, hipStream_t stream);
-// Predicts P4, P3, P5 for every entry of the scatter grid of one atmosphere (batched forward passes).
+// Predicts P4, P3, P5 for every entry of the scatter grid of one atmosphere: layer-1 tables, then one
+// fused kernel for layers 2-4 and the decoding.
 hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* p4,
                                     float4* p3, float4* p5, hipStream_t stream);
 
@@ -103,6 +109,10 @@ hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const Atmo
 hipError_t mfma_gemm(const uint16_t* A, long long sam, long long sak, const uint16_t* B, long long sbk,
                      long long sbn, float* C, int ldc, const float* bias, int act, uint16_t* out_bf16, int M, int N,
                      int K, hipStream_t stream);
+// The same product with K split over up to kMaxSplits workgroups per tile (C row-major, ldc = N);
+// partial holds kMaxSplits x M x N floats.
+hipError_t mfma_gemm_split_k(const uint16_t* A, long long sam, long long sak, const uint16_t* B, long long sbk,
+                             long long sbn, float* C, int M, int N, int K, float* partial, hipStream_t stream);
 
 AtmosphereCode atmosphere_code(const double* rayleigh_base, double rayleigh_scale, double mie_base, double mie_scale,
                                double mie_g, double albedo);

@@ -209,6 +209,42 @@ def test_full_single_scatter_subset(oracle, full):
     assert_close("M1", M1.reshape(-1, 3)[idx], oracle.stage_single_scatter(OP, 1, idx), TOL_SCATTER)
 
 
+def _grazing_entries(shape, rows):
+    """Entries of the 4D grid whose view ray grazes the ground sphere: elevation index 0 (the ray
+    tangent to the ground at the horizon point), the two indices next to the ground/sky split and the
+    last index (the longest ray to the top, whose extension touches the ground sphere); height index 0
+    holds the horizontal rays (mu = 0 at h = 0).  All sun elevations and headings of the given rows."""
+    NH, NE, NL, NA = shape
+    elev = sorted({0, 1, NE // 2 - 1, NE // 2, NE - 2, NE - 1})
+    ih, ie, k = np.meshgrid(rows, elev, np.arange(NL * NA), indexing="ij")
+    return np.sort(((ih * NE + ie) * NL * NA + k).reshape(-1))
+
+
+def test_full_grazing_rays(oracle, full):
+    """Transmittance and single scatter on grazing and tangent view rays (the FP64 segment path)."""
+    P, T, R1, M1 = full[0], full[1], full[3], full[4]
+    OP = oracle_params(oracle, P)
+    NT = tuple(P.transmittance_shape)
+    tidx = np.array([h * NT[1] + e for h in range(NT[0]) for e in sorted({0, 1, NT[1] // 2 - 1, NT[1] // 2,
+                                                                            NT[1] - 2, NT[1] - 1})])
+    assert_close("T grazing", T.reshape(-1, 3)[tidx], oracle.stage_transmittance(OP, tidx), TOL_TRANSMITTANCE)
+    idx = _grazing_entries(tuple(P.scatter_shape), [0, 1, 7, 31])
+    assert_close("R1 grazing", R1.reshape(-1, 3)[idx], oracle.stage_single_scatter(OP, 0, idx), TOL_SCATTER)
+    assert_close("M1 grazing", M1.reshape(-1, 3)[idx], oracle.stage_single_scatter(OP, 1, idx), TOL_SCATTER)
+
+
+def test_production_grazing_rays(atm, gpu, oracle):
+    """The reference's production grid (35 km, T 64x255, S 32x127x32x8, 100 steps): odd elevation sizes
+    put an index exactly on the ground/sky split (a vertical ray by the reference's limit-quot)."""
+    P = atm.workload("production")
+    OP = oracle_params(oracle, P)
+    assert_close("T production", gpu.transmittance_table(P), oracle.stage_transmittance(OP), TOL_TRANSMITTANCE)
+    R1, M1 = gpu.single_scatter_tables(P)
+    idx = _grazing_entries(tuple(P.scatter_shape), [0, 3, 31])
+    assert_close("R1 production", R1.reshape(-1, 3)[idx], oracle.stage_single_scatter(OP, 0, idx), TOL_SCATTER)
+    assert_close("M1 production", M1.reshape(-1, 3)[idx], oracle.stage_single_scatter(OP, 1, idx), TOL_SCATTER)
+
+
 def test_full_point_scatter_subset(oracle, full):
     P, T, E0, R1, M1, J1, dE1, dS1, J2 = full
     OP = oracle_params(oracle, P)
