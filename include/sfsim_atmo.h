@@ -138,10 +138,10 @@ int sfsim_atmo_count_work(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, unsig
 /* ---- Learned stand-in for scattering orders >= 2 (no counterpart in the reference) -------------
  * A small MLP (4 LUT coordinates + 6 atmosphere parameters, 8-band Fourier features, 3 x 128 SiLU,
  * bf16 matrix-core products with FP32 accumulation, Adam in FP32) trained on tables that the plain
- * GPU iteration produces.  It predicts the sums P_k = dS_1 + ... + dS_k (k = 3, 4, 5) of the
- * scattering tables; the seeded build applies one step of the plain iteration to them and falls back
- * to the plain iteration when the network's own P5 differs from that step's result by more than the
- * tolerance. */
+ * GPU iteration produces.  It predicts the order sums P_k = dS_1 + ... + dS_k (k = 1..5) of the
+ * scattering tables; the seeded build applies the iteration (atmosphere_lut.clj:86-97) m times to the
+ * predicted partial sum of 5 - m orders and falls back to the plain iteration when the network's own
+ * P_5 differs from the result by more than the tolerance. */
 size_t sfsim_atmo_surrogate_size(void); /* number of FP32 parameters */
 /* (Re)initialise the network deterministically from seed; batch: training / inference rows per pass. */
 int sfsim_atmo_surrogate_reset(sfsim_atmo_ctx* ctx, unsigned long long seed, int batch);
@@ -149,7 +149,7 @@ int sfsim_atmo_surrogate_weights(sfsim_atmo_ctx* ctx, float* out, size_t count);
 /* trained != 0 marks the weights as usable by sfsim_atmo_build_seeded. */
 int sfsim_atmo_surrogate_set_weights(sfsim_atmo_ctx* ctx, const float* in, size_t count, int trained);
 /* Forward pass over rows of 10 inputs in [0, 1] (outputs: rows x 16); with targets (rows x 16, first
- * 9 used) also the loss (mean over rows of half the squared error) and its gradient (size() floats). */
+ * 15 used) also the loss (mean over rows of half the squared error) and its gradient (size() floats). */
 int sfsim_atmo_surrogate_eval(sfsim_atmo_ctx* ctx, const float* inputs, const float* targets, int rows,
                               float* outputs, float* grad, float* loss);
 /* Trains on the 5-order plain builds of `count` atmospheres sharing one scatter shape: `steps` Adam
@@ -157,13 +157,19 @@ int sfsim_atmo_surrogate_eval(sfsim_atmo_ctx* ctx, const float* inputs, const fl
  * step. */
 int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atmospheres, int count, int steps,
                                double lr, unsigned long long seed, float* losses);
-/* Device-resident 5-order build seeded by the network (outputs as sfsim_atmo_build, read with
- * sfsim_atmo_fetch); residual receives the relative residual, fallback 1 when the plain iteration ran. */
-int sfsim_atmo_build_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tolerance, double* residual,
-                            int* fallback);
-/* The sums P3, P4, P5 of the plain iteration, and the network's prediction of them (scatter tables, RGB). */
-int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* p3, float* p4, float* p5);
-int sfsim_atmo_surrogate_predict(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* p3, float* p4, float* p5);
+/* Device-resident 5-order build seeded by the network with `applications` (1..4) exact applications
+ * of the iteration (outputs as sfsim_atmo_build, read with sfsim_atmo_fetch); residual receives the
+ * relative residual, fallback 1 when the plain iteration ran instead. */
+int sfsim_atmo_build_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int applications, double tolerance,
+                            double* residual, int* fallback);
+/* The same with the order sums P_1..P_5 supplied by the caller (5 RGB scatter tables one after the
+ * other, e.g. those of a neighbouring atmosphere in a parameter sweep) instead of the network's. */
+int sfsim_atmo_build_from_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int applications, const float* sums,
+                               double tolerance, double* residual, int* fallback);
+/* The order sums P_1..P_5 of the plain iteration, and the network's prediction of them: sums holds
+ * 5 scatter tables (RGB) one after the other. */
+int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* sums);
+int sfsim_atmo_surrogate_predict(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* sums);
 
 /* util.clj:227-240 spit-floats / util.clj:188-203 slurp-floats: raw little-endian float32, no header. */
 int sfsim_atmo_write_scatter(const char* path, const float* data, size_t count);

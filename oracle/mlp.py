@@ -9,7 +9,8 @@ sfsim_amd/csrc/surrogate.hip:
 * layers 192 -> 128 -> 128 -> 128 -> 16, SiLU after the three hidden layers, linear output;
 * parameters, flattened in this order: W1 [192 x 128] (row = input feature), b1 [128], W2 [128 x 128],
   b2, W3 [128 x 128], b3, W4 [128 x 16], b4 [16];
-* loss = mean over rows of sum over the first NOUT outputs of mask * (y - target)^2 / 2.
+* loss = mean over rows of sum over the outputs of mask * (y - target)^2 / 2 (the library uses the
+  first 15 outputs: log order sums P_1..P_5, RGB).
 """
 import numpy as np
 

@@ -29,7 +29,8 @@ EXPORTS = (
     "sfsim_atmo_stage_name", "sfsim_atmo_count_work", "sfsim_atmo_write_scatter", "sfsim_atmo_read_scatter",
     "sfsim_atmo_sphere_directions", "sfsim_atmo_surrogate_size", "sfsim_atmo_surrogate_reset",
     "sfsim_atmo_surrogate_weights", "sfsim_atmo_surrogate_set_weights", "sfsim_atmo_surrogate_eval",
-    "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_order_sums", "sfsim_atmo_surrogate_predict",
+    "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_build_from_sums", "sfsim_atmo_order_sums",
+    "sfsim_atmo_surrogate_predict",
 )
 
 
@@ -156,10 +157,12 @@ def lib():
         L.sfsim_atmo_surrogate_eval.argtypes = [vp, fp, fp, ctypes.c_int, fp, fp, fp]
         L.sfsim_atmo_surrogate_train.argtypes = [vp, pp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong,
                                                  fp]
-        L.sfsim_atmo_build_seeded.argtypes = [vp, pp, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+        L.sfsim_atmo_build_seeded.argtypes = [vp, pp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int)]
-        L.sfsim_atmo_order_sums.argtypes = [vp, pp, fp, fp, fp]
-        L.sfsim_atmo_surrogate_predict.argtypes = [vp, pp, fp, fp, fp]
+        L.sfsim_atmo_build_from_sums.argtypes = [vp, pp, ctypes.c_int, fp, ctypes.c_double,
+                                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+        L.sfsim_atmo_order_sums.argtypes = [vp, pp, fp]
+        L.sfsim_atmo_surrogate_predict.argtypes = [vp, pp, fp]
         _lib = L
     return _lib
 
@@ -373,25 +376,41 @@ class Surrogate:
         return losses[:steps]
 
     def predict(self, P):
-        shape = tuple(P.scatter_shape) + (3,)
-        out = [np.zeros(shape, np.float32) for _ in range(3)]
-        self.ctx._check(lib().sfsim_atmo_surrogate_predict(self.ctx._ctx, ctypes.byref(P), *[_fp(o) for o in out]))
-        return out  # P3, P4, P5
+        """The network's order sums P_1..P_5 over the scatter grid of P: array [5, *scatter_shape, 3]."""
+        out = np.zeros((ORDERS,) + tuple(P.scatter_shape) + (3,), np.float32)
+        self.ctx._check(lib().sfsim_atmo_surrogate_predict(self.ctx._ctx, ctypes.byref(P), _fp(out)))
+        return out
+
+
+ORDERS = 5
 
 
 def order_sums(ctx, P):
-    """P3, P4, P5: sums of the plain iteration's scattering tables dS_1..dS_k."""
-    shape = tuple(P.scatter_shape) + (3,)
-    out = [np.zeros(shape, np.float32) for _ in range(3)]
-    ctx._check(lib().sfsim_atmo_order_sums(ctx._ctx, ctypes.byref(P), *[_fp(o) for o in out]))
+    """P_k = dS_1 + ... + dS_k (k = 1..5) of the plain iteration: array [5, *scatter_shape, 3]."""
+    out = np.zeros((ORDERS,) + tuple(P.scatter_shape) + (3,), np.float32)
+    ctx._check(lib().sfsim_atmo_order_sums(ctx._ctx, ctypes.byref(P), _fp(out)))
     return out
 
 
-def build_seeded(ctx, P, tolerance=1e-3):
-    """sfsim_atmo_build_seeded: returns (residual, fell_back)."""
+def build_seeded(ctx, P, applications=1, tolerance=1e-3):
+    """sfsim_atmo_build_seeded: the 5-order set from the network's partial sum of 5 - applications
+    orders and `applications` exact steps of the iteration; returns (residual, fell_back)."""
     r = ctypes.c_double()
     f = ctypes.c_int()
-    ctx._check(lib().sfsim_atmo_build_seeded(ctx._ctx, ctypes.byref(P), tolerance, ctypes.byref(r), ctypes.byref(f)))
+    ctx._check(lib().sfsim_atmo_build_seeded(ctx._ctx, ctypes.byref(P), applications, tolerance, ctypes.byref(r),
+                                             ctypes.byref(f)))
+    return r.value, bool(f.value)
+
+
+def build_from_sums(ctx, P, sums, applications=1, tolerance=1e-3):
+    """sfsim_atmo_build_from_sums: as build_seeded with caller-supplied order sums [5, *scatter_shape, 3]."""
+    arr = np.ascontiguousarray(np.asarray(sums, np.float32))
+    if arr.size != ORDERS * int(np.prod(P.scatter_shape)) * 3:
+        raise AtmoError(ERR_ARGUMENT, "sums must hold 5 scatter tables")
+    r = ctypes.c_double()
+    f = ctypes.c_int()
+    ctx._check(lib().sfsim_atmo_build_from_sums(ctx._ctx, ctypes.byref(P), applications, _fp(arr), tolerance,
+                                                ctypes.byref(r), ctypes.byref(f)))
     return r.value, bool(f.value)
 
 

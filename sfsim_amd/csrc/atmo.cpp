@@ -64,7 +64,7 @@ struct sfsim_atmo_ctx {
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
   Surrogate sg;
-  DeviceBuffer codes, targets, sumS, cap[3], pred[3], resid;
+  DeviceBuffer codes, targets, sumS, cap[kMlpOrders], pred[kMlpOrders], resid;
   double last_residual = 0.0;
   // timing
   std::vector<hipEvent_t> events;
@@ -172,6 +172,39 @@ int validate_half_sphere(sfsim_atmo_ctx* ctx, int steps) {
   return 0;
 }
 
+bool finite_floats(const float* v, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+// Cubic through the 4 Chebyshev nodes of [0, xh], xh = (Rt^2 - R^2) / R^2, of
+// f(x) = k (R / 2) p(x), p(x) = 2 (sqrt(1 + x) - 1) / x = 2 / (1 + sqrt(1 + x)), so that
+// k h = x f(x) for the altitude h = sqrt(R^2 + x R^2) - R.  Relative error of the fit: 4e-10 for a
+// 100 km atmosphere on Earth, 4e-8 at the largest xh = 0.1 the parameters may have.
+void altitude_fit(double R, double Rt, double k, double out[4]) {
+  const double xh = (Rt * Rt - R * R) / (R * R);
+  double A[4][5];
+  for (int i = 0; i < 4; ++i) {
+    const double x = 0.5 * xh * (1.0 + std::cos((2 * i + 1) * M_PI / 8.0));
+    double v = 1.0;
+    for (int j = 0; j < 4; ++j, v *= x) A[i][j] = v;
+    A[i][4] = k * 0.5 * R * 2.0 / (1.0 + std::sqrt(1.0 + x));
+  }
+  for (int c = 0; c < 4; ++c) {  // Gauss-Jordan with partial pivoting
+    int piv = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+    for (int j = 0; j < 5; ++j) std::swap(A[c][j], A[piv][j]);
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const double f = A[r][c] / A[c][c];
+      for (int j = c; j < 5; ++j) A[r][j] -= f * A[c][j];
+    }
+  }
+  for (int j = 0; j < 4; ++j) out[j] = A[j][4] / A[j][j];
+}
+
 DevParams make_dev_params(const sfsim_atmo_params* p) {
   DevParams P;
   std::memset(&P, 0, sizeof(P));
@@ -190,9 +223,10 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   }
   P.k_mie = (float)(-log2e / p->mie.scatter_scale);
   P.k_ray = (float)(-log2e / p->rayleigh.scatter_scale);
-  P.k_mie_2R = (float)(-log2e / p->mie.scatter_scale / (2.0 * P.g.R));
-  P.k_ray_2R = (float)(-log2e / p->rayleigh.scatter_scale / (2.0 * P.g.R));
-  P.inv_R2f = (float)(1.0 / (P.g.R * P.g.R));
+  P.inv_R = 1.0 / P.g.R;
+  P.ray_over_mie = (float)(p->mie.scatter_scale / p->rayleigh.scatter_scale);
+  altitude_fit(P.g.R, P.g.Rt, -log2e / p->mie.scatter_scale, P.alt_poly_d);
+  for (int j = 0; j < 4; ++j) P.alt_poly[j] = (float)P.alt_poly_d[j];
   P.inv_scale_mie = 1.0 / p->mie.scatter_scale;
   P.inv_scale_ray = 1.0 / p->rayleigh.scatter_scale;
   P.g_mie = (float)p->mie.scatter_g;
@@ -335,8 +369,8 @@ struct Timer {
   } while (0)
 
 // The whole driver of atmosphere_lut.clj:43-105 on device buffers of the context.
-// capture (nullable): receives the sums dS_1..dS_k of the scattering tables for k = 3, 4, 5 (float4
-// tables), the training targets of the surrogate.
+// capture (nullable): capture[k - 1] receives the sum dS_1 + ... + dS_k of the scattering tables
+// (float4 tables, k = 1..min(iterations, 5)), the training targets of the surrogate.
 int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* wc, bool timed,
                  float4* const* capture = nullptr) {
   TRY(validate(ctx, p));
@@ -404,9 +438,9 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
         HIP_TRY(ctx, hipMemcpyAsync(sum, d, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
       else
         HIP_TRY(ctx, launch_add4(sum, d, sum, (long long)nS, ctx->stream));
-      const int order = it + 1;  // capture[0] = k 3, capture[1] = k 4, capture[2] = k 5
-      if (order >= 3 && order <= 5)
-        HIP_TRY(ctx, hipMemcpyAsync(capture[order - 3], sum, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
+      const int order = it + 1;
+      if (order <= kMlpOrders)
+        HIP_TRY(ctx, hipMemcpyAsync(capture[order - 1], sum, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
     }
     // E <- table(E + dE), S <- table(S + dS) (atmosphere_lut.clj:95-97)
     LAUNCH(timer, kAccumulate,
@@ -439,17 +473,24 @@ AtmosphereCode code_of(const sfsim_atmo_params* p) {
                          p->mie.scatter_scale, p->mie.scatter_g, p->planet.brightness[0]);
 }
 
-// The 5-order set with the network standing in for orders 2-5 of the state the iteration carries
-// (atmosphere_lut.clj:86-97 is the linear recurrence (dS_k, dE_k) = (S[J[dS_{k-1}, dE_{k-1}]], E[dS_{k-1}])):
-// with P_k = dS_1 + ... + dS_k predicted for k = 3, 4, 5,
-//   sum_{k<=4} dS_k = dS_0 + P4,   sum_{k<=4} dE_k = E0 + E[dS_0 + P3],
-// and one application of the recurrence to these sums gives sum_{k=1..5} (dS_k, dE_k), with the
-// network's error multiplied by the recurrence's (contracting) gain.  The network's own P5 against
-// that result is the residual; above `tol` the plain iteration runs instead.
-int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tol, double* residual, int* fallback) {
+// The 5-order set with the network standing in for part of the iteration.  atmosphere_lut.clj:86-97 is
+// the linear recurrence x_k = L x_{k-1} on x_k = (dS_k, dE_k): dS_k = S[J[dS_{k-1}, dE_{k-1}]],
+// dE_k = E[dS_{k-1}], x_0 = (dS_0, E0).  Its partial sums X_j = x_0 + ... + x_j obey
+// X_{j+1} = x_0 + L X_j, and with P_j = dS_1 + ... + dS_j the sum X_j is
+// (dS_0 + P_j, E0 + E[dS_0 + P_{j-1}]).  Seeding X_{5-m} with the network's P_{5-m}, P_{4-m} and applying
+// the recurrence m times gives X_5 with the network's error multiplied by L^m (a contraction);
+// m = `applications` in 1..4.  The network's own P_5 against the computed sum is the residual; above
+// `tol` the plain iteration runs instead.
+// host_sums (nullable): the order sums P_1..P_5 given by the caller (5 RGB scatter tables) instead of
+// the network's.
+int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double tol, double* residual, int* fallback,
+               const float* host_sums = nullptr) {
   TRY(validate(ctx, p));
-  if (p->iterations != 5) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "the surrogate stands in for a 5-order set");
-  if (!ctx->sg.trained) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate is not trained");
+  if (p->iterations != kMlpOrders) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "the surrogate stands in for a 5-order set");
+  if (m < 1 || m > kMlpOrders - 1) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "applications must be in 1..4");
+  if (!host_sums && !ctx->sg.trained) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate is not trained");
+  if (host_sums && !finite_floats(host_sums, (size_t)kMlpOrders * count4(p->scatter_shape) * 3))
+    return fail(ctx, SFSIM_ATMO_ERR_NONFINITE, "order sums hold NaN/Inf");
   TRY(validate_sphere(ctx, p->sphere_steps));
   TRY(validate_half_sphere(ctx, p->ray_steps));
   TRY(ensure_dirs(ctx, 0, p->sphere_steps));
@@ -467,8 +508,9 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tol, doub
   TRY(ensure(ctx, ctx->R1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->M1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->dJ, nS * sizeof(float4)));
-  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
+  for (int i = 0; i < kMlpOrders; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->resid, 4 * sizeof(unsigned)));
+  if (host_sums) TRY(ensure(ctx, ctx->stage_rgb, nS * 3 * sizeof(float)));
   const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
   for (int i = 0; i < 4; ++i) {
     TRY(ensure(ctx, ctx->out[i], outc[i] * sizeof(float)));
@@ -479,37 +521,62 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tol, doub
   auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
   const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
   const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
-  const long long n4 = (long long)nS;
+  const long long n4 = (long long)nS, e4 = (long long)nE;
+  const int j = kMlpOrders - m;  // seeded partial sum X_j, j >= 1
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), nullptr, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), nullptr, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), nullptr, ctx->stream));
   TRY(timer.begin(kDirectionGeometry));
   TRY(prepare_point_scatter(ctx, P, true, nullptr));
   TRY(timer.end());
-  // network: P4, P3, P5
+  // network: P_j, P_{j-1} (seed) and P_5 (residual)
+  float4* want[kMlpOrders] = {};
+  want[j - 1] = f4(ctx->pred[j - 1]);
+  if (j >= 2) want[j - 2] = f4(ctx->pred[j - 2]);
+  want[kMlpOrders - 1] = f4(ctx->pred[kMlpOrders - 1]);
+  if (host_sums) {
+    for (int k = 0; k < kMlpOrders; ++k)
+      if (want[k]) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->stage_rgb.ptr, host_sums + (size_t)k * nS * 3, nS * 3 * sizeof(float),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, launch_unpack_rgb((const float*)ctx->stage_rgb.ptr, want[k], n4, ctx->stream));
+      }
+  } else {
+    TRY(timer.begin(kSurrogate));
+    HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), want, ctx->stream));
+    TRY(timer.end());
+  }
+  // X_j: S part dS_0 + P_j carried as A = R1 + P_j (the Mie phase term enters through M1), E part
+  // E0 + E[dS_0 + P_{j-1}]
+  float4* A = f4(ctx->Sacc[0]);
+  float4* B = f4(ctx->Sacc[1]);
+  float4* Esum = f4(ctx->Eacc[0]);
+  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), want[j - 1], A, n4, ctx->stream));
+  if (j >= 2) {
+    LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), want[j - 2], B, n4, ctx->stream));
+    LAUNCH(timer, kSurfaceRadiance,
+           launch_surface_radiance(P, hdirs, ctx->half_n, B, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
+  } else {
+    LAUNCH(timer, kSurfaceRadiance, launch_surface_radiance(P, hdirs, ctx->half_n, f4(ctx->R1), f4(ctx->M1),
+                                                            f4(ctx->dE[1]), nullptr, ctx->stream));
+  }
+  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->dE[0]), f4(ctx->dE[1]), Esum, e4, ctx->stream));
+  // m applications of X <- x_0 + L X; after the last, dS[0] = P_5 and dE[1] = dE_1 + ... + dE_5
+  for (int a = 0; a < m; ++a) {
+    LAUNCH(timer, kPointScatter,
+           launch_point_scatter(P, sdirs, ctx->sphere_n, ctx->geom.ptr, A, f4(ctx->M1), Esum, f4(ctx->slab),
+                                f4(ctx->slab_mie), f4(ctx->dJ), nullptr, ctx->stream));
+    LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[0]), nullptr, ctx->stream));
+    LAUNCH(timer, kSurfaceRadiance,
+           launch_surface_radiance(P, hdirs, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
+    if (a + 1 < m) {
+      LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->dS[0]), A, n4, ctx->stream));
+      LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->dE[0]), f4(ctx->dE[1]), Esum, e4, ctx->stream));
+    }
+  }
+  // residual of the network's own P_5, relative to R1 + P_5
   TRY(timer.begin(kSurrogate));
-  HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), f4(ctx->pred[0]), f4(ctx->pred[1]), f4(ctx->pred[2]),
-                                        ctx->stream));
-  TRY(timer.end());
-  float4* A = f4(ctx->Sacc[0]);  // R1 + P4  (dS_0 + P4 without the Mie phase term)
-  float4* B = f4(ctx->Sacc[1]);  // R1 + P3
-  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->pred[0]), A, n4, ctx->stream));
-  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->pred[1]), B, n4, ctx->stream));
-  // sum_{k<=4} dE_k = E0 + E[dS_0 + P3]
-  LAUNCH(timer, kSurfaceRadiance,
-         launch_surface_radiance(P, hdirs, ctx->half_n, B, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
-  LAUNCH(timer, kAccumulate, launch_resample(P, 3, f4(ctx->dE[0]), f4(ctx->dE[1]), f4(ctx->Eacc[0]), nullptr, 0,
-                                             ctx->stream));
-  // one application: sum_{k=1..5} dS_k = S[J[sum dS, sum dE]], sum_{k=1..5} dE_k = E[sum dS]
-  LAUNCH(timer, kPointScatter,
-         launch_point_scatter(P, sdirs, ctx->sphere_n, ctx->geom.ptr, A, f4(ctx->M1), f4(ctx->Eacc[0]), f4(ctx->slab),
-                              f4(ctx->slab_mie), f4(ctx->dJ), nullptr, ctx->stream));
-  LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[0]), nullptr, ctx->stream));
-  LAUNCH(timer, kSurfaceRadiance,
-         launch_surface_radiance(P, hdirs, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
-  // residual of the network's own 5-order sum, relative to R1 + sum
-  TRY(timer.begin(kSurrogate));
-  HIP_TRY(ctx, launch_residual(f4(ctx->R1), f4(ctx->dS[0]), f4(ctx->pred[2]), n4, 1e-6f, (unsigned*)ctx->resid.ptr,
+  HIP_TRY(ctx, launch_residual(f4(ctx->R1), f4(ctx->dS[0]), want[kMlpOrders - 1], n4, 1e-6f, (unsigned*)ctx->resid.ptr,
                                ctx->stream));
   TRY(timer.end());
   unsigned bits[4];
@@ -538,11 +605,6 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tol, doub
 }
 
 // ------------------------------------------------------------------ stage entry helpers
-bool finite_floats(const float* v, size_t n) {
-  for (size_t i = 0; i < n; ++i)
-    if (!std::isfinite(v[i])) return false;
-  return true;
-}
 
 // Upload a host RGB table into float4 staging buffer `slot`.
 int upload_table(sfsim_atmo_ctx* ctx, int slot, const float* host, size_t entries, const float4** dev) {
@@ -611,8 +673,9 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
                           &ctx->stage_rgb, &ctx->sphere_dirs, &ctx->half_dirs, &ctx->counters,
                           &ctx->geom,   &ctx->slab,    &ctx->slab_mie};
   for (DeviceBuffer* b : bufs) release(*b);
-  DeviceBuffer* more[] = {&ctx->codes, &ctx->targets, &ctx->sumS, &ctx->cap[0], &ctx->cap[1], &ctx->cap[2],
-                          &ctx->pred[0], &ctx->pred[1], &ctx->pred[2], &ctx->resid};
+  DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
+                          &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -880,16 +943,17 @@ int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atm
   }
   const size_t nS = (size_t)count4(p0.scatter_shape);
   TRY(ensure(ctx, ctx->sumS, nS * sizeof(float4)));
-  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->cap[i], nS * sizeof(float4)));
+  for (int i = 0; i < kMlpOrders; ++i) TRY(ensure(ctx, ctx->cap[i], nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->targets, (size_t)count * nS * kMlpUsed * sizeof(float)));
   TRY(ensure(ctx, ctx->codes, (size_t)count * sizeof(AtmosphereCode)));
   std::vector<AtmosphereCode> codes(count);
-  // training data: the plain GPU iteration of every atmosphere (orders 1..5, sums k = 3, 4, 5)
+  // training data: the plain GPU iteration of every atmosphere (order sums P_1..P_5)
+  float4* caps[kMlpOrders];
+  for (int k = 0; k < kMlpOrders; ++k) caps[k] = (float4*)ctx->cap[k].ptr;
   for (int a = 0; a < count; ++a) {
-    float4* caps[3] = {(float4*)ctx->cap[0].ptr, (float4*)ctx->cap[1].ptr, (float4*)ctx->cap[2].ptr};
     TRY(run_pipeline(ctx, &atmospheres[a], nullptr, false, caps));
-    HIP_TRY(ctx, surrogate_make_targets(caps[1], caps[0], caps[2], (long long)nS,
-                                        (float*)ctx->targets.ptr + (size_t)a * nS * kMlpUsed, ctx->stream));
+    HIP_TRY(ctx, surrogate_make_targets(caps, (long long)nS, (float*)ctx->targets.ptr + (size_t)a * nS * kMlpUsed,
+                                        ctx->stream));
     codes[a] = code_of(&atmospheres[a]);
   }
   HIP_TRY(ctx, hipMemcpyAsync(ctx->codes.ptr, codes.data(), count * sizeof(AtmosphereCode), hipMemcpyHostToDevice,
@@ -920,43 +984,61 @@ int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atm
   return 0;
 }
 
-int sfsim_atmo_build_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, double tolerance, double* residual,
-                            int* fallback) {
+int sfsim_atmo_build_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int applications, double tolerance,
+                            double* residual, int* fallback) {
   TRY(activate(ctx));
   double r = 0;
   int f = 0;
-  TRY(run_seeded(ctx, p, tolerance, &r, &f));
+  TRY(run_seeded(ctx, p, applications, tolerance, &r, &f));
   if (residual) *residual = r;
   if (fallback) *fallback = f;
   return 0;
 }
 
-int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* p3, float* p4, float* p5) {
+int sfsim_atmo_build_from_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int applications, const float* sums,
+                               double tolerance, double* residual, int* fallback) {
   TRY(activate(ctx));
-  TRY(validate(ctx, p));
-  if (p->iterations != 5) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "order sums need a 5-order build");
-  const size_t nS = (size_t)count4(p->scatter_shape);
-  TRY(ensure(ctx, ctx->sumS, nS * sizeof(float4)));
-  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->cap[i], nS * sizeof(float4)));
-  float4* caps[3] = {(float4*)ctx->cap[0].ptr, (float4*)ctx->cap[1].ptr, (float4*)ctx->cap[2].ptr};
-  TRY(run_pipeline(ctx, p, nullptr, false, caps));
-  TRY(download_table(ctx, caps[0], p3, nS));
-  TRY(download_table(ctx, caps[1], p4, nS));
-  return download_table(ctx, caps[2], p5, nS);
+  if (!sums) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sums is NULL");
+  double r = 0;
+  int f = 0;
+  TRY(run_seeded(ctx, p, applications, tolerance, &r, &f, sums));
+  if (residual) *residual = r;
+  if (fallback) *fallback = f;
+  return 0;
 }
 
-int sfsim_atmo_surrogate_predict(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* p3, float* p4, float* p5) {
+int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* sums) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (p->iterations != kMlpOrders) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "order sums need a 5-order build");
+  if (!sums) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sums is NULL");
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  TRY(ensure(ctx, ctx->sumS, nS * sizeof(float4)));
+  float4* caps[kMlpOrders];
+  for (int k = 0; k < kMlpOrders; ++k) {
+    TRY(ensure(ctx, ctx->cap[k], nS * sizeof(float4)));
+    caps[k] = (float4*)ctx->cap[k].ptr;
+  }
+  TRY(run_pipeline(ctx, p, nullptr, false, caps));
+  for (int k = 0; k < kMlpOrders; ++k) TRY(download_table(ctx, caps[k], sums + (size_t)k * nS * 3, nS));
+  return 0;
+}
+
+int sfsim_atmo_surrogate_predict(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* sums) {
   TRY(activate(ctx));
   TRY(validate(ctx, p));
   if (!ctx->sg.theta) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate not initialised");
+  if (!sums) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sums is NULL");
   const size_t nS = (size_t)count4(p->scatter_shape);
-  for (int i = 0; i < 3; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
+  float4* out[kMlpOrders];
+  for (int k = 0; k < kMlpOrders; ++k) {
+    TRY(ensure(ctx, ctx->pred[k], nS * sizeof(float4)));
+    out[k] = (float4*)ctx->pred[k].ptr;
+  }
   const DevParams P = make_dev_params(p);
-  HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), (float4*)ctx->pred[0].ptr, (float4*)ctx->pred[1].ptr,
-                                        (float4*)ctx->pred[2].ptr, ctx->stream));
-  TRY(download_table(ctx, (const float4*)ctx->pred[1].ptr, p3, nS));
-  TRY(download_table(ctx, (const float4*)ctx->pred[0].ptr, p4, nS));
-  return download_table(ctx, (const float4*)ctx->pred[2].ptr, p5, nS);
+  HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), out, ctx->stream));
+  for (int k = 0; k < kMlpOrders; ++k) TRY(download_table(ctx, out[k], sums + (size_t)k * nS * 3, nS));
+  return 0;
 }
 
 int sfsim_atmo_write_scatter(const char* path, const float* data, size_t count) {

@@ -15,8 +15,13 @@ struct DevParams {
   float sca_mie[3], sca_ray[3];
   // exp(-h / scale) = exp2(h * k): k = -log2(e) / scale
   float k_mie, k_ray;
-  float k_mie_2R, k_ray_2R;  // k / 2R (altitude series of the FP32 sums)
-  float inv_R2f;             // 1 / R^2
+  // FP32 sums in units of the planet radius: with x = (r^2 - R^2) / R^2 the Mie exponent is
+  // h k_mie = x (a0 + x (a1 + x (a2 + x a3))), a cubic fit at Chebyshev nodes on [0, (Rt^2 - R^2) / R^2]
+  // of (R / 2) k_mie 2 (sqrt(1 + x) - 1) / x; the Rayleigh exponent is ray_over_mie times it
+  float alt_poly[4];
+  double alt_poly_d[4];  // the same cubic in FP64 (FP64 sums)
+  float ray_over_mie;
+  double inv_R;
   double inv_scale_mie, inv_scale_ray;
   float g_mie, g_ray;
   float sun_inv_den;  // 1 / (1 - exp(-3.6)) of sun-elevation-to-index (atmosphere.clj:322-326)

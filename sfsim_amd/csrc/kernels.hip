@@ -43,8 +43,8 @@ __device__ __forceinline__ float phase_f(float g, float mu) {
 // cancellation: near ts, where the density is largest, u ~ u(ts) is exact, and elsewhere s and
 // b' + s have the same sign.  h = u / (R + sqrt(R^2 + u)).
 struct Segment {
-  float c0, b, s0, dt;  // FP32 node constants: s_j = j dt + s0, u_j = c0 + s_j (b + s_j)
-  double c0d, bd, s0d, dtd;
+  float c0, b, s0, dt;  // FP32 node constants in units of R: x_j = c0 + s_j (b + s_j), s_j = j dt + s0
+  double c0d, bd, s0d, dtd;  // the same in metres (u_j = c0d + s_j (bd + s_j)) for the FP64 sums
   bool graze;           // lowest point interior and below the FP64 altitude threshold
 };
 
@@ -55,37 +55,40 @@ __device__ __forceinline__ Segment make_segment(const DevParams& P, double h0, d
   g.bd = 2.0 * pd0 + 2.0 * ts;
   g.dtd = D / n;
   g.s0d = 0.5 * g.dtd - ts;
-  g.c0 = (float)g.c0d;
-  g.b = (float)g.bd;
-  g.s0 = (float)g.s0d;
-  g.dt = (float)g.dtd;
+  g.c0 = (float)(g.c0d * P.inv_R * P.inv_R);
+  g.b = (float)(g.bd * P.inv_R);
+  g.s0 = (float)(g.s0d * P.inv_R);
+  g.dt = (float)(g.dtd * P.inv_R);
   g.graze = (-pd0 > 0.0) && (-pd0 < D) && (g.c0d < P.graze_u);
   return g;
 }
 
 // Sums over the n midpoint nodes of exp(-h / H_mie) and exp(-h / H_rayleigh) with FP32 altitudes;
-// 8-node partial sums, Kahan-compensated across them.  The altitude h = u / (R + sqrt(R^2 + u)) is
-// evaluated as (u / 2R) p(u / R^2) with the series p(x) = 2 (sqrt(1 + x) - 1) / x to x^4 (relative
-// truncation < 5e-7 at the largest x = (Rt^2 - R^2) / R^2 = 0.1 the host accepts, < 2e-8 for a
-// 100 km atmosphere), so a node costs FMAs and the two exponentials only; 1 / 2R is folded into
-// the exponent scales km2, kr2.
+// 8-node partial sums, Kahan-compensated across them.  Node j sits at s_j = s0 + j dt (units of R,
+// measured from the segment's lowest point), x_j = c0 + s_j (b + s_j) = (r_j^2 - R^2) / R^2, and the
+// Mie exponent log2(exp(-h_j / H_mie)) is x_j times a cubic in x_j (DevParams::alt_poly), so a node
+// costs five FMAs, two multiplies, an add and the two exponentials.
 __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) {
-  const float km = P.k_mie_2R, kr = P.k_ray_2R, inv_R2 = P.inv_R2f;
+  const float a0 = P.alt_poly[0], a1 = P.alt_poly[1], a2 = P.alt_poly[2], a3 = P.alt_poly[3];
+  const float rho = P.ray_over_mie;
   float sm = 0.f, sr = 0.f, cm = 0.f, cr = 0.f;
-  float jf = 0.f;
   int j = 0;
   for (; j + 8 <= n; j += 8) {
+    const float base = fmaf((float)j, g.dt, g.s0);
     float pm = 0.f, pr = 0.f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      float s = fmaf(jf, g.dt, g.s0);
-      float u = fmaf(s, s + g.b, g.c0);
-      float x = u * inv_R2;
-      float p = fmaf(fmaf(fmaf(fmaf(0.0546875f, x, -0.078125f), x, 0.125f), x, -0.25f), x, 1.0f);
-      float up = u * p;
-      pm += fexp2(up * km);
-      pr += fexp2(up * kr);
-      jf += 1.0f;
+      const float s = fmaf((float)q, g.dt, base);
+      const float x = fmaf(s, s + g.b, g.c0);
+      const float em = x * fmaf(fmaf(fmaf(a3, x, a2), x, a1), x, a0);
+      const float m = fexp2(em), r = fexp2(em * rho);
+      if (q == 0) {
+        pm = m;
+        pr = r;
+      } else {
+        pm += m;
+        pr += r;
+      }
     }
     kahan_add(sm, cm, pm);
     kahan_add(sr, cr, pr);
@@ -93,14 +96,11 @@ __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) 
   if (j < n) {
     float pm = 0.f, pr = 0.f;
     for (; j < n; ++j) {
-      float s = fmaf(jf, g.dt, g.s0);
-      float u = fmaf(s, s + g.b, g.c0);
-      float x = u * inv_R2;
-      float p = fmaf(fmaf(fmaf(fmaf(0.0546875f, x, -0.078125f), x, 0.125f), x, -0.25f), x, 1.0f);
-      float up = u * p;
-      pm += fexp2(up * km);
-      pr += fexp2(up * kr);
-      jf += 1.0f;
+      const float s = fmaf((float)j, g.dt, g.s0);
+      const float x = fmaf(s, s + g.b, g.c0);
+      const float em = x * fmaf(fmaf(fmaf(a3, x, a2), x, a1), x, a0);
+      pm += fexp2(em);
+      pr += fexp2(em * rho);
     }
     kahan_add(sm, cm, pm);
     kahan_add(sr, cr, pr);
@@ -108,34 +108,40 @@ __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) 
   return make_float2(sm, sr);
 }
 
-// Same sums with FP64 node positions, altitudes and accumulation (near-horizon and grazing
-// segments).  h = (u / 2R) * 2 (sqrt(1 + x) - 1) / x, x = u / R^2, by its Taylor series
-// (|x| <= (Rt^2 - R^2) / R^2 <= 0.1 is checked on the host; 10 terms leave < 2e-12 relative);
-// the exponentials are FP32 (their argument carries the altitude's FP32 rounding, 6e-8).
+// Same sums with FP64 node positions, altitude exponents and accumulation (near-horizon and grazing
+// segments): x_j = u_j / R^2 from the metre-valued constants and the FP64 cubic of DevParams; the
+// exponentials are FP32 (their argument is rounded to FP32, a relative 6e-8).
 __device__ double2 density_sums_f64(const DevParams& P, const Segment& g, int n) {
-  const double inv_R2 = 1.0 / (P.g.R * P.g.R), inv_2R = 0.5 / P.g.R;
-  const float km = P.k_mie, kr = P.k_ray;
+  const double inv_R2 = P.inv_R * P.inv_R;
+  const double a0 = P.alt_poly_d[0], a1 = P.alt_poly_d[1], a2 = P.alt_poly_d[2], a3 = P.alt_poly_d[3];
+  const float rho = P.ray_over_mie;
   double sm = 0.0, sr = 0.0;
   for (int j = 0; j < n; ++j) {
-    double s = fma((double)j, g.dtd, g.s0d);
-    double u = fma(s, s + g.bd, g.c0d);
-    double x = u * inv_R2;
-    // 2 (sqrt(1 + x) - 1) / x = sum_k 2 binom(1/2, k + 1) x^k, k = 0..9
-    double p = -0.01854705810546875;
-    p = fma(p, x, 0.021820068359375);
-    p = fma(p, x, -0.02618408203125);
-    p = fma(p, x, 0.0322265625);
-    p = fma(p, x, -0.041015625);
-    p = fma(p, x, 0.0546875);
-    p = fma(p, x, -0.078125);
-    p = fma(p, x, 0.125);
-    p = fma(p, x, -0.25);
-    p = fma(p, x, 1.0);
-    float h = (float)(u * inv_2R * p);
-    sm += (double)fexp2(h * km);
-    sr += (double)fexp2(h * kr);
+    const double s = fma((double)j, g.dtd, g.s0d);
+    const double x = fma(s, s + g.bd, g.c0d) * inv_R2;
+    const float em = (float)(x * fma(fma(fma(a3, x, a2), x, a1), x, a0));
+    sm += (double)fexp2(em);
+    sr += (double)fexp2(em * rho);
   }
   return make_double2(sm, sr);
+}
+
+// exp(-(beta_mie * integral of the Mie density + beta_rayleigh * ...)) per channel from the two sums.
+__device__ __forceinline__ float3 transmittance_of(const DevParams& P, float sm, float sr, double D) {
+  if (!(D > 0.0)) return make_float3(1.f, 1.f, 1.f);
+  return make_float3(expf(-(P.ext_mie[0] * sm + P.ext_ray[0] * sr)), expf(-(P.ext_mie[1] * sm + P.ext_ray[1] * sr)),
+                     expf(-(P.ext_mie[2] * sm + P.ext_ray[2] * sr)));
+}
+
+__device__ __forceinline__ float3 segment_transmittance_f32(const DevParams& P, const Segment& g, double D, int n) {
+  float2 s = density_sums_f32(P, g, n);
+  const float dt = (float)g.dtd;
+  return transmittance_of(P, s.x * dt, s.y * dt, D);
+}
+
+__device__ __forceinline__ float3 segment_transmittance_f64(const DevParams& P, const Segment& g, double D, int n) {
+  double2 s = density_sums_f64(P, g, n);
+  return transmittance_of(P, (float)(s.x * g.dtd), (float)(s.y * g.dtd), D);
 }
 
 // Transmittance of the segment of length D from altitude h0 along a unit direction with
@@ -144,19 +150,8 @@ __device__ double2 density_sums_f64(const DevParams& P, const Segment& g, int n)
 // diverge within a wave.
 __device__ float3 segment_transmittance(const DevParams& P, double h0, double pd0, double D, int n) {
   Segment g = make_segment(P, h0, pd0, D, n);
-  float sm, sr;
-  if (__any(g.graze)) {
-    double2 s = density_sums_f64(P, g, n);
-    sm = (float)(s.x * g.dtd);
-    sr = (float)(s.y * g.dtd);
-  } else {
-    float2 s = density_sums_f32(P, g, n);
-    sm = s.x * g.dt;
-    sr = s.y * g.dt;
-  }
-  if (!(D > 0.0)) return make_float3(1.f, 1.f, 1.f);
-  return make_float3(expf(-(P.ext_mie[0] * sm + P.ext_ray[0] * sr)), expf(-(P.ext_mie[1] * sm + P.ext_ray[1] * sr)),
-                     expf(-(P.ext_mie[2] * sm + P.ext_ray[2] * sr)));
+  if (__any(g.graze)) return segment_transmittance_f64(P, g, D, n);
+  return segment_transmittance_f32(P, g, D, n);
 }
 
 // ------------------------------------------------------------------ multilinear lookups
@@ -351,48 +346,77 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
   unsigned long long samples = 0;
   for (int i0 = 0; i0 < n; i0 += kSSChunk) {
     const int cn = min(kSSChunk, n - i0);
-    unsigned mask = 0;
+    // lit samples (atmosphere.clj:154-160), split by the optical-depth path their sun segment takes:
+    // FP64 when its lowest point (at distance -dot(p_i, l) when that is positive) lies below
+    // graze_height, FP32 otherwise
+    unsigned m32 = 0, m64 = 0;
     if (valid)
       for (int c = 0; c < cn; ++c) {
-        int i = i0 + c;
-        if (is_above_horizon(P.g, s_h[i], xl + s_t[i] * vl)) mask |= 1u << c;  // atmosphere.clj:154-160
+        const int i = i0 + c;
+        const double h = s_h[i], pl = xl + s_t[i] * vl;
+        if (is_above_horizon(P.g, h, pl)) {
+          if (pl < 0.0 && h * (2.0 * P.g.R + h) - pl * pl < P.graze_u)
+            m64 |= 1u << c;
+          else
+            m32 |= 1u << c;
+        }
       }
-    const int cnt = __popc(mask);
-    const int off = wave_exclusive_scan(cnt, lane);
-    const int total = __shfl(off + cnt, kSSWave - 1, 64);
+    const int c32 = __popc(m32), c64 = __popc(m64);
+    const int o32 = wave_exclusive_scan(c32, lane), o64 = wave_exclusive_scan(c64, lane);
+    const int t32 = __shfl(o32 + c32, kSSWave - 1, 64), t64 = __shfl(o64 + c64, kSSWave - 1, 64);
     {
-      unsigned m = mask;
-      for (int q = 0; q < cnt; ++q) {
+      unsigned m = m32;
+      for (int q = 0; q < c32; ++q) {
         int c = __ffs(m) - 1;
         m &= m - 1;
-        s_item[off + q] = (unsigned short)((lane << 8) | c);
+        s_item[o32 + q] = (unsigned short)((lane << 8) | c);
+      }
+      m = m64;
+      for (int q = 0; q < c64; ++q) {
+        int c = __ffs(m) - 1;
+        m &= m - 1;
+        s_item[t32 + o64 + q] = (unsigned short)((lane << 8) | c);
       }
     }
     __syncthreads();
-    for (int it = lane; it < total; it += kSSWave) {
-      unsigned item = s_item[it];
-      int owner = item >> 8, i = i0 + (int)(item & 255);
-      double h = s_h[i];
-      double pl = s_xl[owner] + s_t[i] * s_vl[owner];  // dot(p_i, l)
-      double Ds = top_exit_distance(P.g, h, pl);
-      float3 Ts = segment_transmittance(P, h, pl, Ds, n);
+    // sun transmittance T(p_i, l) of every listed item: the FP32 list, then the FP64 list, so the two
+    // loops never share a wavefront
+    for (int it = lane; it < t32; it += kSSWave) {
+      const unsigned item = s_item[it];
+      const int owner = item >> 8, i = i0 + (int)(item & 255);
+      const double h = s_h[i];
+      const double pl = s_xl[owner] + s_t[i] * s_vl[owner];  // dot(p_i, l)
+      const double Ds = top_exit_distance(P.g, h, pl);
+      const float3 Ts = segment_transmittance_f32(P, make_segment(P, h, pl, Ds, n), Ds, n);
+      s_res[it] = make_float4(Ts.x, Ts.y, Ts.z, 0.f);
+    }
+    for (int it = t32 + lane; it < t32 + t64; it += kSSWave) {
+      const unsigned item = s_item[it];
+      const int owner = item >> 8, i = i0 + (int)(item & 255);
+      const double h = s_h[i];
+      const double pl = s_xl[owner] + s_t[i] * s_vl[owner];
+      const double Ds = top_exit_distance(P.g, h, pl);
+      const float3 Ts = segment_transmittance_f64(P, make_segment(P, h, pl, Ds, n), Ds, n);
       s_res[it] = make_float4(Ts.x, Ts.y, Ts.z, 0.f);
     }
     __syncthreads();
     {
-      unsigned m = mask;
-      for (int q = 0; q < cnt; ++q) {
-        int c = __ffs(m) - 1;
+      // each entry adds its lit samples in sample order (deterministic sums)
+      unsigned m = m32 | m64;
+      int r32 = o32, r64 = t32 + o64;
+      while (m) {
+        const int c = __ffs(m) - 1;
         m &= m - 1;
-        float4 Ts = s_res[off + q];
-        float4 w = s_w[i0 + c];
-        float br = s_bray[i0 + c];
-        float3 t3 = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
+        const int idx = ((m32 >> c) & 1u) ? r32++ : r64++;
+        const float4 Ts = s_res[idx];
+        const float4 w = s_w[i0 + c];
+        const float br = s_bray[i0 + c];
+        const float3 t3 = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
         accR.x += t3.x * br; accR.y += t3.y * br; accR.z += t3.z * br;
         accM.x += t3.x * w.w; accM.y += t3.y * w.w; accM.z += t3.z * w.w;
       }
     }
-    samples += (unsigned long long)cnt * n;
+    samples += (unsigned long long)(c32 + c64) * n;
     __syncthreads();
   }
   if (valid) {

@@ -139,13 +139,19 @@ __global__ void gather_kernel(const DevParams P, const AtmosphereCode* __restric
   for (int o = 0; o < kMlpOut; ++o) T[(long long)r * kMlpOut + o] = o < kMlpUsed ? t[o] : 0.f;
 }
 
-__global__ void targets_kernel(const float4* __restrict__ p4, const float4* __restrict__ p3,
-                               const float4* __restrict__ p5, long long entries, float* __restrict__ t) {
+struct SumTables {
+  const float4* p[kMlpOrders];
+};
+
+__global__ void targets_kernel(const SumTables sums, long long entries, float* __restrict__ t) {
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= entries) return;
-  float4 a = p4[e], b = p3[e], c = p5[e];
-  float v[kMlpUsed] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
-  for (int o = 0; o < kMlpUsed; ++o) t[e * kMlpUsed + o] = logf(fmaxf(v[o], 0.f) + kMlpEps) + kMlpShift;
+  for (int k = 0; k < kMlpOrders; ++k) {
+    const float4 v = sums.p[k][e];
+    t[e * kMlpUsed + 3 * k + 0] = logf(fmaxf(v.x, 0.f) + kMlpEps) + kMlpShift;
+    t[e * kMlpUsed + 3 * k + 1] = logf(fmaxf(v.y, 0.f) + kMlpEps) + kMlpShift;
+    t[e * kMlpUsed + 3 * k + 2] = logf(fmaxf(v.z, 0.f) + kMlpEps) + kMlpShift;
+  }
 }
 
 // dY = mask (Y - T) / rows (FP32 and bf16), loss partials 0.5 sum d^2 / rows per block.
@@ -267,10 +273,13 @@ __device__ __forceinline__ short4_t silu_bf16x4(float a, float b, float c, float
 // 8 g + 4 (l / 32) + i (g = 0..3, i = 0..3) of entry l % 32, which is exactly the lane's B operand of
 // the next layer's K-steps 8 g .. 8 g + 7, so activations never leave the registers.  W2^T, W3^T and
 // W4^T (padded to 32 rows) are staged once per workgroup in LDS; workgroups loop over the tiles.
+struct PredTables {
+  float4* p[kMlpOrders];
+};
+
 __global__ void __launch_bounds__(64 * kFusedWaves) fused_predict_kernel(
     const DevParams P, const uint16_t* __restrict__ wbf, const float* __restrict__ theta,
-    const float* __restrict__ ztab, long long entries, float4* __restrict__ p4, float4* __restrict__ p3,
-    float4* __restrict__ p5) {
+    const float* __restrict__ ztab, long long entries, const PredTables out) {
   __shared__ __align__(16) uint16_t lds[kFusedLds / 2];
   uint16_t* w2t = lds;
   uint16_t* w3t = w2t + kMlpHidden * kLdsRow;
@@ -347,26 +356,26 @@ __global__ void __launch_bounds__(64 * kFusedWaves) fused_predict_kernel(
       const short4_t a = *(const short4_t*)(w4t + r * kLdsRow + 8 * s + 4 * h);
       y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[s], y, 0, 0, 0);
     }
-    // lane (r, h) holds outputs 4 h + i (y[i]) and 8 + 4 h + i (y[4 + i]) of entry r
-    float o[kMlpUsed];
-    const float u0 = __shfl(y[0], r + 32, 64), u1 = __shfl(y[1], r + 32, 64), u2 = __shfl(y[2], r + 32, 64),
-                u3 = __shfl(y[3], r + 32, 64);
-    o[0] = y[0];
-    o[1] = y[1];
-    o[2] = y[2];
-    o[3] = y[3];
-    o[4] = u0;
-    o[5] = u1;
-    o[6] = u2;
-    o[7] = u3;
-    o[8] = y[4];
-    if (h == 0 && e < entries) {
-      float v[kMlpUsed];
+    // lane (r, h) holds outputs 4 h + i (y[i]) and 8 + 4 h + i (y[4 + i]) of entry r, i = 0..3
+    float o[kMlpOut];
 #pragma unroll
-      for (int k = 0; k < kMlpUsed; ++k) v[k] = fmaxf(expf(o[k] + b4[k] - kMlpShift) - kMlpEps, 0.f);
-      p4[e] = make_float4(v[0], v[1], v[2], 0.f);
-      p3[e] = make_float4(v[3], v[4], v[5], 0.f);
-      p5[e] = make_float4(v[6], v[7], v[8], 0.f);
+    for (int i = 0; i < 4; ++i) {
+      const float u = __shfl(y[i], r + 32, 64), v = __shfl(y[4 + i], r + 32, 64);
+      o[i] = y[i];
+      o[4 + i] = u;
+      o[8 + i] = y[4 + i];
+      o[12 + i] = v;
+    }
+    if (h == 0 && e < entries) {
+#pragma unroll
+      for (int k = 0; k < kMlpOrders; ++k) {
+        if (out.p[k]) {
+          float v[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[c] = fmaxf(expf(o[3 * k + c] + b4[3 * k + c] - kMlpShift) - kMlpEps, 0.f);
+          out.p[k][e] = make_float4(v[0], v[1], v[2], 0.f);
+        }
+      }
     }
   }
 }
@@ -558,14 +567,15 @@ hipError_t surrogate_gather_batch(Surrogate& s, const DevParams& P, const Atmosp
   return hipGetLastError();
 }
 
-hipError_t surrogate_make_targets(const float4* p4, const float4* p3, const float4* p5, long long entries,
-                                  float* targets_dev, hipStream_t stream) {
-  hipLaunchKernelGGL(targets_kernel, dim3(ew_blocks(entries)), dim3(kEw), 0, stream, p4, p3, p5, entries, targets_dev);
+hipError_t surrogate_make_targets(const float4* const* sums, long long entries, float* targets_dev, hipStream_t stream) {
+  SumTables st;
+  for (int k = 0; k < kMlpOrders; ++k) st.p[k] = sums[k];
+  hipLaunchKernelGGL(targets_kernel, dim3(ew_blocks(entries)), dim3(kEw), 0, stream, st, entries, targets_dev);
   return hipGetLastError();
 }
 
-hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* p4,
-                                    float4* p3, float4* p5, hipStream_t stream) {
+hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* const* out,
+                                    hipStream_t stream) {
   const long long entries = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
   const int rows = 1 + P.nS[0] + P.nS[1] + P.nS[2] + P.nS[3];
   if ((size_t)rows * kMlpHidden > s.ztab_capacity) {
@@ -581,8 +591,10 @@ hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const Atmo
   const long long tiles = (entries + 31) / 32;
   const long long groups = (tiles + kFusedWaves - 1) / kFusedWaves;
   const unsigned grid = (unsigned)std::min<long long>(groups, 1024);
+  PredTables pt;
+  for (int k = 0; k < kMlpOrders; ++k) pt.p[k] = out[k];
   hipLaunchKernelGGL(fused_predict_kernel, dim3(grid), dim3(64 * kFusedWaves), 0, stream, P, s.theta_bf,
-                     s.theta, s.ztab, entries, p4, p3, p5);
+                     s.theta, s.ztab, entries, pt);
   return hipGetLastError();
 }
 

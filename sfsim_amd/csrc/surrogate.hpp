@@ -2,9 +2,9 @@
 //
 // Input: 4 LUT coordinates of an entry of the scatter grid (index / (size - 1)) and 6 atmosphere
 // parameters mapped to [0, 1]; 8-band Fourier features (x, sin 2^b pi x, cos 2^b pi x), 170 values
-// padded to 192; three hidden layers of 128 with SiLU; 16 outputs of which the first 9 are used:
-// log(P + eps) + 10 for P = sum of the scattering tables dS_1..dS_k of the plain iteration for k = 4, 3
-// and 5 (RGB each).  Weights are kept in FP32 (Adam state too); every matrix product runs as bf16 x bf16
+// padded to 192; three hidden layers of 128 with SiLU; 16 outputs of which the first 15 are used:
+// output 3 (k - 1) + c is log(P_k + eps) + 10 for channel c of P_k = dS_1 + ... + dS_k, the sum of the
+// first k scattering tables of the plain iteration (k = 1..5).  Weights are kept in FP32 (Adam state too); every matrix product runs as bf16 x bf16
 // with FP32 accumulation on the matrix cores (v_mfma_f32_32x32x8bf16_1k).
 #pragma once
 
@@ -22,7 +22,8 @@ constexpr int kMlpBands = 17;  // features per input: x, sin and cos of 2^b pi x
 constexpr int kMlpFeat = 192;
 constexpr int kMlpHidden = 128;
 constexpr int kMlpOut = 16;
-constexpr int kMlpUsed = 9;
+constexpr int kMlpOrders = 5;
+constexpr int kMlpUsed = 3 * kMlpOrders;
 constexpr float kMlpEps = 1e-8f;     // floor inside the log
 constexpr float kMlpShift = 10.0f;   // targets are log(P + eps) + shift
 constexpr int kMaxSplits = 32;       // K splits of the weight-gradient products
@@ -95,13 +96,13 @@ hipError_t surrogate_adam(Surrogate& s, float lr, hipStream_t stream);
 // `entries` scatter-grid entries each; targets[atm][entry][9] hold log(P + eps) + shift.
 hipError_t surrogate_gather_batch(Surrogate& s, const DevParams& P, const AtmosphereCode* codes_dev, int natm,
                                   const float* targets_dev, uint64_t seed, int step, int rows, hipStream_t stream);
-// Targets of one atmosphere from its sums P3, P4, P5 of the plain iteration (float4 tables).
-hipError_t surrogate_make_targets(const float4* p4, const float4* p3, const float4* p5, long long entries,
-                                  float* targets_dev, hipStream_t stream);
-// Predicts P4, P3, P5 for every entry of the scatter grid of one atmosphere: layer-1 tables, then one
-// fused kernel for layers 2-4 and the decoding.
-hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* p4,
-                                    float4* p3, float4* p5, hipStream_t stream);
+// Targets of one atmosphere from its order sums P_1..P_5 (float4 tables).
+hipError_t surrogate_make_targets(const float4* const* sums, long long entries, float* targets_dev, hipStream_t stream);
+// Predicts the order sums P_k for every entry of the scatter grid of one atmosphere into out[k - 1]
+// (entries of `out` may be null: not written): layer-1 tables, then one fused kernel for layers 2-4
+// and the decoding.
+hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* const* out,
+                                    hipStream_t stream);
 
 // Matrix product on the matrix cores: C[m][n] = sum_k A(m, k) B(k, n) (+ bias[n]), A(m, k) at
 // A[m * sam + k * sak], B(k, n) at B[k * sbk + n * sbn] (bf16), C row-major FP32 with leading dim ldc.

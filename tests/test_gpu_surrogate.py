@@ -106,10 +106,10 @@ def test_loss_and_gradient_match_fp64_network(net):
     sg, theta, x = net
     rng = np.random.default_rng(7)
     T = np.zeros((x.shape[0], 16), np.float32)
-    T[:, :9] = rng.normal(0.0, 1.0, (x.shape[0], 9))
+    T[:, :15] = rng.normal(0.0, 1.0, (x.shape[0], 15))
     y, g, loss = sg.eval(x, T)
     mask = np.zeros((x.shape[0], 16))
-    mask[:, :9] = 1.0
+    mask[:, :15] = 1.0
     ref_loss, ref_g = mlp.loss_and_grad(theta, mlp.encode(x), T.astype(np.float64), mask)
     assert loss == pytest.approx(ref_loss, rel=2e-2)
     offsets = np.cumsum([0] + [int(np.prod(s)) for s in mlp.SHAPES])
@@ -120,9 +120,9 @@ def test_loss_and_gradient_match_fp64_network(net):
         cos = gg @ rr / (np.linalg.norm(gg) * np.linalg.norm(rr))
         print(f"{name}: relative error {rel:.2e}, cosine {cos:.6f}")
         assert rel <= 3e-2 and cos >= 0.999, name
-    # outputs 9..15 are masked: their weight and bias gradients are exactly zero
+    # output 15 is masked: its weight and bias gradients are exactly zero
     W4g = g[offsets[6]:offsets[7]].reshape(128, 16)
-    assert not W4g[:, 9:].any() and not g[offsets[7] + 9:offsets[8]].any()
+    assert not W4g[:, 15:].any() and not g[offsets[7] + 15:offsets[8]].any()
 
 
 def atmosphere_code(P):
@@ -144,20 +144,21 @@ def entry_inputs(P, entries):
 
 
 def layerwise_prediction(sg, P, entries, batch):
-    """P4, P3, P5 of the given entries through the training path's products (sfsim_atmo_surrogate_eval)."""
+    """P_1..P_5 of the given entries through the training path's products (sfsim_atmo_surrogate_eval):
+    array [5, len(entries), 3]."""
     x = entry_inputs(P, entries)
     y = np.concatenate([sg.eval(x[i:i + batch])[0] for i in range(0, len(x), batch)])
-    v = np.maximum(np.exp(y[:, :9].astype(np.float64) - 10.0) - 1e-8, 0.0)
-    return v[:, 0:3], v[:, 3:6], v[:, 6:9]
+    v = np.maximum(np.exp(y[:, :15].astype(np.float64) - 10.0) - 1e-8, 0.0)
+    return v.reshape(-1, 5, 3).transpose(1, 0, 2)
 
 
 def assert_same_prediction(fused, layerwise):
     """Both paths round to bf16 at the same places; only FP32 summation order differs, which can flip
     an activation's last bf16 bit: about 1e-3 in the log outputs at worst."""
-    for name, a, b in zip(("P4", "P3", "P5"), fused, layerwise):
-        d = np.abs(np.log(a.astype(np.float64) + 1e-8) - np.log(b + 1e-8))
-        print(f"{name}: max |d log| {d.max():.2e}, median {np.median(d):.2e}")
-        assert d.max() <= 2e-2 and np.median(d) <= 2e-3, name
+    for k in range(5):
+        d = np.abs(np.log(fused[k].astype(np.float64) + 1e-8) - np.log(layerwise[k] + 1e-8))
+        print(f"P{k + 1}: max |d log| {d.max():.2e}, median {np.median(d):.2e}")
+        assert d.max() <= 2e-2 and np.median(d) <= 2e-3, k + 1
 
 
 def test_fused_prediction_matches_layerwise_full_grid(atm, net):
@@ -166,12 +167,11 @@ def test_fused_prediction_matches_layerwise_full_grid(atm, net):
     sg, theta, _ = net
     sg.set_weights(theta.astype(np.float32), trained=False)
     P = atm.workload("full-5-order")
-    q3, q4, q5 = sg.predict(P)
+    q = sg.predict(P)
     n = int(np.prod(P.scatter_shape))
     entries = np.sort(np.random.default_rng(9).choice(n, 2048, replace=False))
     entries[-1] = n - 1
-    got = [q.reshape(n, 3)[entries] for q in (q4, q3, q5)]
-    assert_same_prediction(got, layerwise_prediction(sg, P, entries, 256))
+    assert_same_prediction(q.reshape(5, n, 3)[:, entries], layerwise_prediction(sg, P, entries, 256))
 
 
 def _oracle_order_sums(oracle, OP):
@@ -187,16 +187,16 @@ def _oracle_order_sums(oracle, OP):
         dS = oracle.stage_ray_scatter(OP, J)
         total = total + dS
         sums.append(total)
-    return sums[2], sums[3], sums[4]
+    return sums
 
 
 def test_order_sums_match_oracle(atm, gpu, oracle):
     P = small5(atm)
-    p3, p4, p5 = atm.order_sums(gpu, P)
-    r3, r4, r5 = _oracle_order_sums(oracle, oracle_params(oracle, P))
-    for name, got, ref in (("P3", p3, r3), ("P4", p4, r4), ("P5", p5, r5)):
-        assert_close(name, got, ref, TOL_SCATTER)
-    assert (p3 >= 0).all() and (p4 >= p3).all() and (p5 >= p4).all()
+    got = atm.order_sums(gpu, P)
+    ref = _oracle_order_sums(oracle, oracle_params(oracle, P))
+    for k in range(5):
+        assert_close(f"P{k + 1}", got[k], ref[k], TOL_SCATTER)
+    assert (got[0] >= 0).all() and all((got[k + 1] >= got[k]).all() for k in range(4))
 
 
 @pytest.fixture(scope="module")
@@ -233,18 +233,17 @@ def test_training_is_deterministic(atm, gpu, trained):
 def test_fused_prediction_matches_layerwise_small_grid(atm, trained):
     """A grid whose entry count (1848) is not a multiple of the 32-entry tile."""
     sg, P, family, _ = trained
-    q3, q4, q5 = sg.predict(P)
     n = int(np.prod(P.scatter_shape))
-    assert_same_prediction([q.reshape(n, 3) for q in (q4, q3, q5)], layerwise_prediction(sg, P, np.arange(n), 1024))
+    assert_same_prediction(sg.predict(P).reshape(5, n, 3), layerwise_prediction(sg, P, np.arange(n), 1024))
 
 
 def test_prediction_tracks_order_sums(atm, gpu, trained):
     sg, P, family, _ = trained
-    p3, p4, p5 = atm.order_sums(gpu, P)
-    q3, q4, q5 = sg.predict(P)
-    for name, got, ref in (("P3", q3, p3), ("P4", q4, p4), ("P5", q5, p5)):
-        err = rel_errors(got, ref)
-        print(f"{name}: median {np.median(err):.3e}, max {err.max():.3e}")
+    sums = atm.order_sums(gpu, P)
+    pred = sg.predict(P)
+    for k in range(5):
+        err = rel_errors(pred[k], sums[k])
+        print(f"P{k + 1}: median {np.median(err):.3e}, max {err.max():.3e}")
         assert np.median(err) < 0.1
 
 
@@ -253,37 +252,53 @@ def _error_stats(got, ref):
     return {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e * e))), "median": float(np.median(e))}
 
 
-def test_seeded_build_beats_the_step_without_network(atm, gpu, trained):
-    """What the trained network gains: the seeded build against the same single exact step taken from
-    a zero seed (two scattering orders instead of five), both measured against the plain 5-order build
-    (surface radiance and ray scatter tables)."""
+@pytest.mark.parametrize("applications", [1, 2])
+def test_seeded_build_beats_the_steps_without_network(atm, gpu, trained, applications):
+    """What the trained network gains: the seeded build against the same exact steps taken from a zero
+    seed (applications + 1 scattering orders instead of five), both measured against the plain 5-order
+    build (surface radiance and ray scatter tables)."""
     sg, P, family, _ = trained
     gpu.build(P)
     plain = [gpu.fetch(P, i) for i in range(4)]
-    r, fell_back = atm.build_seeded(gpu, P, tolerance=np.inf)
+    r, fell_back = atm.build_seeded(gpu, P, applications, tolerance=np.inf)
     assert not fell_back and np.isfinite(r) and r >= 0
     seeded = [gpu.fetch(P, i) for i in range(4)]
     assert np.array_equal(seeded[0], plain[0]) and np.array_equal(seeded[3], plain[3])
     w = sg.weights()
     zero = np.zeros_like(w)
-    zero[-16:-7] = 10.0 + np.log(1e-8)  # decodes to P = 0
+    zero[-16:-1] = 10.0 + np.log(1e-8)  # decodes to P = 0
     sg.set_weights(zero, trained=True)
     try:
-        atm.build_seeded(gpu, P, tolerance=np.inf)
+        atm.build_seeded(gpu, P, applications, tolerance=np.inf)
         trunc = [gpu.fetch(P, i) for i in range(4)]
     finally:
         sg.set_weights(w, trained=True)
     es = _error_stats(seeded[1:3], plain[1:3])
     et = _error_stats(trunc[1:3], plain[1:3])
-    print(f"seeded {es} (residual {r:.3e}); without network {et}")
+    print(f"m={applications}: seeded {es} (residual {r:.3e}); without network {et}")
     assert es["rms"] < 0.5 * et["rms"] and es["median"] < 0.5 * et["median"]
+
+
+@pytest.mark.parametrize("applications", [1, 2, 4])
+def test_seeded_build_with_exact_sums_is_the_plain_build(atm, gpu, applications):
+    """Seeded with the plain iteration's own order sums, m applications reproduce the plain 5-order
+    build up to FP32 rounding (the accumulations add instead of re-sampling the tables) and the
+    residual vanishes to the same level: the seeded algebra is exact."""
+    P = small5(atm)
+    gpu.build(P)
+    plain = [gpu.fetch(P, i) for i in range(4)]
+    sums = atm.order_sums(gpu, P)
+    r, fell_back = atm.build_from_sums(gpu, P, sums, applications, tolerance=1e-4)
+    assert not fell_back and r <= 1e-4
+    for i, name in enumerate(("transmittance", "surface-radiance", "ray-scatter", "mie-strength")):
+        assert_close(name, gpu.fetch(P, i), plain[i], 1e-4)
 
 
 def test_seeded_build_falls_back_to_plain(atm, gpu, trained):
     sg, P, family, _ = trained
     gpu.build(P)
     plain = [gpu.fetch(P, i) for i in range(4)]
-    r, fell_back = atm.build_seeded(gpu, P, tolerance=0.0)
+    r, fell_back = atm.build_seeded(gpu, P, 2, tolerance=0.0)
     assert r > 0 and fell_back
     for i in range(4):
         assert np.array_equal(gpu.fetch(P, i), plain[i])
@@ -297,6 +312,8 @@ def test_seeded_build_refuses_untrained_network(atm):
             atm.build_seeded(ctx, small5(atm))
         with pytest.raises(atm.AtmoError):
             atm.build_seeded(ctx, small5(atm, iterations=3))
+        with pytest.raises(atm.AtmoError):
+            atm.build_seeded(ctx, small5(atm), applications=5)
     finally:
         ctx.close()
 
