@@ -4,7 +4,9 @@ A step is one complete LUT build (transmittance, surface radiance, ray scatter a
 5 scattering orders) of the full 5-order workload: T 64x256, E 16x64, S/M 32x64x32x8, 64 ray steps,
 14 sphere steps (66 directions), 100 km atmosphere.  Its inputs are the parameters alone, so the
 timed region starts with nothing but parameters on the host and ends with the four final tables in
-device memory (sfsim_atmo_build; copying them out is not part of a step).
+device memory (sfsim_atmo_build, which replays the build's captured HIP graph; copying the tables
+out is not part of a step).  Per-kernel durations come from the same number of kernel-by-kernel
+builds with HIP events (sfsim_atmo_build_timed) after the timed steps.
 
 Prints one JSON line with the throughput (LUT entries/s), the roofline of the dominant kernel
 (single scattering, measured with HIP events on the library's stream) and the FP64 CPU oracle timed
@@ -121,18 +123,21 @@ def main():
     from sfsim_amd import atmosphere as atm
     P = atm.workload(args.workload)
     ctx = atm.Atmosphere(0)
-    for _ in range(args.warmup):
-        ctx.build(P)
-    kernel_ms = {}
+    for _ in range(max(args.warmup, 1)):
+        ctx.build(P)  # the first build captures the graph that the timed steps replay
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.build(P)  # synchronous: returns after the stream has drained
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    # per-kernel durations: the same build launched kernel by kernel with HIP events on its stream
+    kernel_ms = {}
+    for _ in range(args.steps):
+        ctx.build_timed(P)
         for kind, (ms, n) in ctx.stage_timings().items():
             a = kernel_ms.setdefault(kind, [0.0, 0])
             a[0] += ms
             a[1] += n
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
     n_entries = atm.entries(P)
     value = n_entries / (ms_per_step / 1e3)
     errors = check_against_fixture(atm, ctx, P, args.workload)

@@ -120,12 +120,17 @@ int sfsim_atmo_generate_luts(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, fl
 /* Device-resident build for callers that keep the tables on the GPU (and for timing): runs the
  * whole generate-atmosphere-luts pipeline into context-owned device buffers without host copies;
  * sfsim_atmo_fetch then copies one final table out (which: 0 = transmittance, 1 = surface
- * radiance, 2 = ray scatter, 3 = mie strength; layouts as generate_luts). */
+ * radiance, 2 = ray scatter, 3 = mie strength; layouts as generate_luts).  The launches are captured
+ * into a HIP graph on the first build with given parameters and replayed by later builds with the
+ * same parameters (same results: the kernels are deterministic). */
 int sfsim_atmo_build(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p);
+/* The same build launched kernel by kernel with HIP events around each launch (fills the timings
+ * that sfsim_atmo_stage_timings reports). */
+int sfsim_atmo_build_timed(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p);
 int sfsim_atmo_fetch(sfsim_atmo_ctx* ctx, int which, float* out, size_t count);
 
-/* Kernel durations of the last sfsim_atmo_build, measured with HIP events on the context stream
- * and summed per kernel kind: ms[k] milliseconds and launches[k] launches of kind k, for
+/* Kernel durations of the last sfsim_atmo_build_timed (or seeded build), measured with HIP events
+ * on the context stream and summed per kernel kind: ms[k] milliseconds and launches[k] launches of kind k, for
  * k < min(n, number of kinds); returns the number of kinds.  Kind names: sfsim_atmo_stage_name(k). */
 int sfsim_atmo_stage_timings(const sfsim_atmo_ctx* ctx, double* ms, int* launches, int n);
 const char* sfsim_atmo_stage_name(int kind);

@@ -25,7 +25,7 @@ EXPORTS = (
     "sfsim_atmo_create", "sfsim_atmo_destroy", "sfsim_atmo_last_error", "sfsim_atmo_transmittance_table",
     "sfsim_atmo_surface_radiance_base_table", "sfsim_atmo_single_scatter_tables", "sfsim_atmo_point_scatter_table",
     "sfsim_atmo_ray_scatter_table", "sfsim_atmo_surface_radiance_table", "sfsim_atmo_resample_table",
-    "sfsim_atmo_generate_luts", "sfsim_atmo_build", "sfsim_atmo_fetch", "sfsim_atmo_stage_timings",
+    "sfsim_atmo_generate_luts", "sfsim_atmo_build", "sfsim_atmo_build_timed", "sfsim_atmo_fetch", "sfsim_atmo_stage_timings",
     "sfsim_atmo_stage_name", "sfsim_atmo_count_work", "sfsim_atmo_write_scatter", "sfsim_atmo_read_scatter",
     "sfsim_atmo_sphere_directions", "sfsim_atmo_surrogate_size", "sfsim_atmo_surrogate_reset",
     "sfsim_atmo_surrogate_weights", "sfsim_atmo_surrogate_set_weights", "sfsim_atmo_surrogate_eval",
@@ -140,6 +140,7 @@ def lib():
         L.sfsim_atmo_resample_table.argtypes = [vp, pp, ctypes.c_int, fp, fp, fp]
         L.sfsim_atmo_generate_luts.argtypes = [vp, pp, fp, fp, fp, fp]
         L.sfsim_atmo_build.argtypes = [vp, pp]
+        L.sfsim_atmo_build_timed.argtypes = [vp, pp]
         L.sfsim_atmo_fetch.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
         L.sfsim_atmo_stage_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                                                ctypes.c_int]
@@ -294,7 +295,12 @@ class Atmosphere:
         return {"transmittance": T, "surface-radiance": E, "ray-scatter": S, "mie-strength": M}
 
     def build(self, P):
+        """sfsim_atmo_build: graph-replayed device-resident build."""
         self._check(lib().sfsim_atmo_build(self._ctx, ctypes.byref(P)))
+
+    def build_timed(self, P):
+        """sfsim_atmo_build_timed: kernel-by-kernel build with per-kernel HIP-event timings."""
+        self._check(lib().sfsim_atmo_build_timed(self._ctx, ctypes.byref(P)))
 
     def fetch(self, P, which):
         s = P.scatter_shape
@@ -306,7 +312,7 @@ class Atmosphere:
         return out
 
     def stage_timings(self):
-        """{kernel kind: (milliseconds, launches)} of the last build()."""
+        """{kernel kind: (milliseconds, launches)} of the last build_timed() or seeded build."""
         ms = (ctypes.c_double * 16)()
         n = (ctypes.c_int * 16)()
         kinds = lib().sfsim_atmo_stage_timings(self._ctx, ms, n, 16)

@@ -66,6 +66,11 @@ struct sfsim_atmo_ctx {
   Surrogate sg;
   DeviceBuffer codes, targets, sumS, cap[kMlpOrders], pred[kMlpOrders], resid;
   double last_residual = 0.0;
+  // graph of the last build() (replayed while parameters and buffers stay the same)
+  hipGraphExec_t graph_exec = nullptr;
+  sfsim_atmo_params graph_params;
+  DevParams graph_dev;
+  unsigned long long alloc_gen = 0, graph_gen = 0;
   // timing
   std::vector<hipEvent_t> events;
   std::vector<int> event_kind;
@@ -99,6 +104,7 @@ int hip_fail(sfsim_atmo_ctx* ctx, const char* what, hipError_t e) {
 
 int ensure(sfsim_atmo_ctx* ctx, DeviceBuffer& b, size_t bytes) {
   if (b.bytes >= bytes && b.ptr) return 0;
+  ctx->alloc_gen++;  // captured graphs hold buffer addresses
   if (b.ptr) {
     (void)hipFree(b.ptr);
     b.ptr = nullptr;
@@ -216,6 +222,8 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   for (int c = 0; c < 3; ++c) {
     P.ext_mie[c] = (float)(p->mie.scatter_base[c] / p->mie.scatter_quotient);
     P.ext_ray[c] = (float)(p->rayleigh.scatter_base[c] / p->rayleigh.scatter_quotient);
+    P.ext_mie_l2[c] = (float)(log2e * p->mie.scatter_base[c] / p->mie.scatter_quotient);
+    P.ext_ray_l2[c] = (float)(log2e * p->rayleigh.scatter_base[c] / p->rayleigh.scatter_quotient);
     P.sca_mie[c] = (float)p->mie.scatter_base[c];
     P.sca_ray[c] = (float)p->rayleigh.scatter_base[c];
     P.brightness_over_pi[c] = (float)(p->planet.brightness[c] / M_PI);
@@ -238,6 +246,7 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   }
   for (int d = 0; d < 4; ++d) P.nS[d] = p->scatter_shape[d];
   P.ray_steps = p->ray_steps;
+  P.inv_steps = 1.0 / p->ray_steps;
   // SFSIM_ATMO_GRAZE_HEIGHT (metres, >= 0) overrides the altitude below which a segment's lowest
   // point selects the FP64 optical-depth path; 0 keeps every segment in FP32.
   P.graze_height = 2000.0;
@@ -368,11 +377,9 @@ struct Timer {
     TRY((timer).end());                                \
   } while (0)
 
-// The whole driver of atmosphere_lut.clj:43-105 on device buffers of the context.
-// capture (nullable): capture[k - 1] receives the sum dS_1 + ... + dS_k of the scattering tables
-// (float4 tables, k = 1..min(iterations, 5)), the training targets of the surrogate.
-int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* wc, bool timed,
-                 float4* const* capture = nullptr) {
+// Validation, quadrature directions and buffers of a build: everything that allocates or copies
+// from the host, so that the launches that follow can be captured into a graph.
+int prepare_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(validate(ctx, p));
   if (p->iterations > 0) {
     TRY(validate_sphere(ctx, p->sphere_steps));
@@ -380,7 +387,6 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
     TRY(ensure_dirs(ctx, 0, p->sphere_steps));
     TRY(ensure_dirs(ctx, 1, p->ray_steps));
   }
-  const DevParams P = make_dev_params(p);
   const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
   const size_t nS = (size_t)count4(p->scatter_shape);
   TRY(ensure(ctx, ctx->T, nT * sizeof(float4)));
@@ -398,8 +404,25 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
     TRY(ensure(ctx, ctx->out[i], outc[i] * sizeof(float)));
     ctx->out_count[i] = outc[i];
   }
+  if (p->iterations > 0) {
+    const size_t slab = nS / ((size_t)p->scatter_shape[1]) * ctx->sphere_n * sizeof(float4);  // NH x dirs x NL x NA
+    TRY(ensure(ctx, ctx->geom, direction_geometry_bytes(make_dev_params(p), ctx->sphere_n)));
+    TRY(ensure(ctx, ctx->slab, slab));
+    TRY(ensure(ctx, ctx->slab_mie, slab));
+  }
   ctx->built = false;
-  Timer timer{ctx, timed};
+  return 0;
+}
+
+// The launches of the whole driver of atmosphere_lut.clj:43-105 on device buffers of the context
+// (prepared by prepare_pipeline): no allocation, host copy or memset, so the sequence can be captured.
+// capture (nullable): capture[k - 1] receives the sum dS_1 + ... + dS_k of the scattering tables
+// (float4 tables, k = 1..min(iterations, 5)), the training targets of the surrogate.
+int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevParams& P, WorkCounters* wc,
+                     Timer& timer, float4* const* capture) {
+  const size_t nE = (size_t)count2(p->surface_radiance_shape);
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  (void)nE;
   auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
   const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
   const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
@@ -407,11 +430,9 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), wc, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), wc, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream));
-  if (p->iterations > 0) {
-    TRY(timer.begin(kDirectionGeometry));
-    TRY(prepare_point_scatter(ctx, P, true, wc));
-    TRY(timer.end());
-  }
+  if (p->iterations > 0)
+    LAUNCH(timer, kDirectionGeometry,
+           launch_direction_geometry(P, sdirs, ctx->sphere_n, ctx->geom.ptr, wc, ctx->stream));
   // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
   const float4* S_cur = f4(ctx->R1);
   const float4* E_cur = nullptr;
@@ -457,12 +478,63 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
   if (E_cur) {
     LAUNCH(timer, kFinal, launch_resample(P, 3, E_cur, nullptr, nullptr, (float*)ctx->out[1].ptr, 0, ctx->stream));
   } else {
-    HIP_TRY(ctx, hipMemsetAsync(ctx->out[1].ptr, 0, outc[1] * sizeof(float), ctx->stream));
+    LAUNCH(timer, kFinal, launch_fill_zero((float*)ctx->out[1].ptr, (long long)ctx->out_count[1], ctx->stream));
   }
   LAUNCH(timer, kFinal, launch_resample(P, 4, S_cur, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
   LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
+  return 0;
+}
+
+// The whole driver, launched eagerly (timed: per-kernel events; wc: work counters).
+int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* wc, bool timed,
+                 float4* const* capture = nullptr) {
+  TRY(prepare_pipeline(ctx, p));
+  const DevParams P = make_dev_params(p);
+  Timer timer{ctx, timed};
+  TRY(enqueue_pipeline(ctx, p, P, wc, timer, capture));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   TRY(timer.collect());
+  ctx->built = true;
+  return 0;
+}
+
+void drop_graph(sfsim_atmo_ctx* ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  ctx->graph_exec = nullptr;
+}
+
+// The whole driver as one graph launch: captured on the first build with these parameters and
+// buffers, replayed afterwards (the kernels are deterministic, so replays equal eager builds).
+int run_pipeline_graph(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  TRY(prepare_pipeline(ctx, p));
+  const DevParams P = make_dev_params(p);
+  const bool hit = ctx->graph_exec && ctx->graph_gen == ctx->alloc_gen &&
+                   std::memcmp(&ctx->graph_params, p, sizeof(*p)) == 0 &&
+                   std::memcmp(&ctx->graph_dev, &P, sizeof(P)) == 0;
+  if (!hit) {
+    drop_graph(ctx);
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    Timer timer{ctx, false};
+    const int rc = enqueue_pipeline(ctx, p, P, nullptr, timer, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != 0) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIP_TRY(ctx, e);
+    const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      ctx->graph_exec = nullptr;
+      return hip_fail(ctx, "hipGraphInstantiate", ei);
+    }
+    ctx->graph_params = *p;
+    ctx->graph_dev = P;
+    ctx->graph_gen = ctx->alloc_gen;
+  }
+  HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->built = true;
   return 0;
 }
@@ -678,6 +750,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
                           &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
+  drop_graph(ctx);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -807,6 +880,11 @@ int sfsim_atmo_resample_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, i
 }
 
 int sfsim_atmo_build(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  TRY(activate(ctx));
+  return run_pipeline_graph(ctx, p);
+}
+
+int sfsim_atmo_build_timed(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(activate(ctx));
   return run_pipeline(ctx, p, nullptr, true);
 }

@@ -12,6 +12,7 @@ struct DevParams {
   Geo g;
   // extinction (scatter-base / scatter-quotient) and scattering (scatter-base) per component, FP32
   float ext_mie[3], ext_ray[3];
+  float ext_mie_l2[3], ext_ray_l2[3];  // the same times log2(e): T = exp2(-(ext_mie_l2 I_mie + ext_ray_l2 I_ray))
   float sca_mie[3], sca_ray[3];
   // exp(-h / scale) = exp2(h * k): k = -log2(e) / scale
   float k_mie, k_ray;
@@ -31,6 +32,7 @@ struct DevParams {
   int nE[2];  // surface radiance shape [height sun-elevation]
   int nS[4];  // scatter shape [height elevation sun-elevation heading]
   int ray_steps;
+  double inv_steps;  // 1 / ray_steps
   // optical depth of a segment whose lowest point lies strictly inside it and below the altitude
   // graze_height (u = r^2 - R^2 < graze_u there) is accumulated in FP64 (near-horizon / grazing
   // rays); everything else in FP32 with compensated sums

@@ -18,10 +18,40 @@ namespace sfsim {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMaxRayChunk = 8;  // samples per slab chunk of the ray-scatter kernel
 
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// FP64 square root and reciprocal from the FP32 hardware estimates and Newton steps in FP64 (relative
+// error ~1e-14, against ~1e-16 for the correctly rounded library sequences that cost several times as
+// many instructions).  Arguments outside the FP32 normal range take the library routines.
+__device__ __forceinline__ double sqrt_newton(double a) {
+  if (!(a > 1e-30 && a < 1e30)) return sqrt(a);
+  double y = (double)__builtin_amdgcn_rsqf((float)a);  // 1 / sqrt(a) to ~1e-7
+  y = y * fma(-0.5 * a * y, y, 1.5);                    // ~1e-14
+  return a * y;
+}
+
+__device__ __forceinline__ double rcp_newton(double a) {
+  if (!(a > 1e-30 && a < 1e30)) return 1.0 / a;
+  double z = (double)__builtin_amdgcn_rcpf((float)a);  // ~1e-7
+  z = z * fma(-a, z, 2.0);                              // ~1e-14
+  return z;
+}
+
+// top_exit_distance (geometry.hpp) with sqrt_newton / rcp_newton.
+__device__ __forceinline__ double top_exit_distance_dev(const Geo& g, double h, double pd) {
+  double c = (g.H - h) * (2.0 * g.R + g.H + h);
+  double disc = pd * pd + c;
+  if (disc > 0) {
+    double l2 = sqrt_newton(disc);
+    double d = pd > 0 ? c * rcp_newton(pd + l2) : l2 - pd;
+    return fmax(0.0, d);
+  }
+  return fmax(0.0, -pd);
+}
 
 __device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
   float y = x - comp;
@@ -53,7 +83,7 @@ __device__ __forceinline__ Segment make_segment(const DevParams& P, double h0, d
   double ts = fmin(fmax(-pd0, 0.0), D);
   g.c0d = h0 * (2.0 * P.g.R + h0) + ts * (2.0 * pd0 + ts);
   g.bd = 2.0 * pd0 + 2.0 * ts;
-  g.dtd = D / n;
+  g.dtd = D * P.inv_steps;  // n == P.ray_steps at every call
   g.s0d = 0.5 * g.dtd - ts;
   g.c0 = (float)(g.c0d * P.inv_R * P.inv_R);
   g.b = (float)(g.bd * P.inv_R);
@@ -126,11 +156,13 @@ __device__ double2 density_sums_f64(const DevParams& P, const Segment& g, int n)
   return make_double2(sm, sr);
 }
 
-// exp(-(beta_mie * integral of the Mie density + beta_rayleigh * ...)) per channel from the two sums.
+// exp(-(beta_mie * integral of the Mie density + beta_rayleigh * ...)) per channel from the two sums,
+// as exp2 of the log2(e)-scaled exponent.
 __device__ __forceinline__ float3 transmittance_of(const DevParams& P, float sm, float sr, double D) {
   if (!(D > 0.0)) return make_float3(1.f, 1.f, 1.f);
-  return make_float3(expf(-(P.ext_mie[0] * sm + P.ext_ray[0] * sr)), expf(-(P.ext_mie[1] * sm + P.ext_ray[1] * sr)),
-                     expf(-(P.ext_mie[2] * sm + P.ext_ray[2] * sr)));
+  return make_float3(fexp2(-fmaf(P.ext_mie_l2[0], sm, P.ext_ray_l2[0] * sr)),
+                     fexp2(-fmaf(P.ext_mie_l2[1], sm, P.ext_ray_l2[1] * sr)),
+                     fexp2(-fmaf(P.ext_mie_l2[2], sm, P.ext_ray_l2[2] * sr)));
 }
 
 __device__ __forceinline__ float3 segment_transmittance_f32(const DevParams& P, const Segment& g, double D, int n) {
@@ -186,6 +218,15 @@ __device__ __forceinline__ float3 lookup2(const float4* __restrict__ T, int N1, 
   float3 u = mix3(ld3(T, (long long)a.i0 * N1 + b.i0), ld3(T, (long long)a.i0 * N1 + b.i1), b.s);
   float3 v = mix3(ld3(T, (long long)a.i1 * N1 + b.i0), ld3(T, (long long)a.i1 * N1 + b.i1), b.s);
   return mix3(u, v, a.s);
+}
+
+// lookup2 when one of the two weights may be exactly zero (a grid coordinate that is an integer):
+// mix(u, v, 0) = u exactly, so the row or column that it would mix in is not read.  Same result as
+// lookup2, bit for bit.
+__device__ __forceinline__ float3 lookup2_skip(const float4* __restrict__ T, int N1, Lerp a, Lerp b) {
+  if (a.s == 0.0f) return mix3(ld3(T, (long long)a.i0 * N1 + b.i0), ld3(T, (long long)a.i0 * N1 + b.i1), b.s);
+  if (b.s == 0.0f) return mix3(ld3(T, (long long)a.i0 * N1 + b.i0), ld3(T, (long long)a.i1 * N1 + b.i0), a.s);
+  return lookup2(T, N1, a, b);
 }
 
 // 4D table [N0][N1][N2][N3], nested like interpolate-value (outer dimension mixed last)
@@ -306,8 +347,8 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
   double* s_h = s_t + n;                         // [n] sample altitude
   float4* s_w = (float4*)(s_h + n);              // [n] T(x, p_i) rgb, w: exp(-h_i / H_mie)
   float* s_bray = (float*)(s_w + n);             // [n] exp(-h_i / H_rayleigh)
-  float4* s_res = (float4*)(s_bray + ((n + 3) & ~3));  // [64 * kSSChunk] sun transmittance per item
-  double* s_xl = (double*)(s_res + kSSWave * kSSChunk);  // [64] dot(x, l) of each lane's entry
+  float* s_res = s_bray + ((n + 3) & ~3);  // [3][64 * kSSChunk] sun transmittance per item (r, g, b planes)
+  double* s_xl = (double*)(s_res + 3 * kSSWave * kSSChunk);  // [64] dot(x, l) of each lane's entry
   double* s_vl = s_xl + kSSWave;                          // [64] dot(v, l)
   unsigned short* s_item = (unsigned short*)(s_vl + kSSWave);  // lane << 8 | chunk sample
 
@@ -386,18 +427,22 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
       const int owner = item >> 8, i = i0 + (int)(item & 255);
       const double h = s_h[i];
       const double pl = s_xl[owner] + s_t[i] * s_vl[owner];  // dot(p_i, l)
-      const double Ds = top_exit_distance(P.g, h, pl);
+      const double Ds = top_exit_distance_dev(P.g, h, pl);
       const float3 Ts = segment_transmittance_f32(P, make_segment(P, h, pl, Ds, n), Ds, n);
-      s_res[it] = make_float4(Ts.x, Ts.y, Ts.z, 0.f);
+      s_res[it] = Ts.x;
+      s_res[it + kSSWave * kSSChunk] = Ts.y;
+      s_res[it + 2 * kSSWave * kSSChunk] = Ts.z;
     }
     for (int it = t32 + lane; it < t32 + t64; it += kSSWave) {
       const unsigned item = s_item[it];
       const int owner = item >> 8, i = i0 + (int)(item & 255);
       const double h = s_h[i];
       const double pl = s_xl[owner] + s_t[i] * s_vl[owner];
-      const double Ds = top_exit_distance(P.g, h, pl);
+      const double Ds = top_exit_distance_dev(P.g, h, pl);
       const float3 Ts = segment_transmittance_f64(P, make_segment(P, h, pl, Ds, n), Ds, n);
-      s_res[it] = make_float4(Ts.x, Ts.y, Ts.z, 0.f);
+      s_res[it] = Ts.x;
+      s_res[it + kSSWave * kSSChunk] = Ts.y;
+      s_res[it + 2 * kSSWave * kSSChunk] = Ts.z;
     }
     __syncthreads();
     {
@@ -408,7 +453,7 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
         const int c = __ffs(m) - 1;
         m &= m - 1;
         const int idx = ((m32 >> c) & 1u) ? r32++ : r64++;
-        const float4 Ts = s_res[idx];
+        const float3 Ts = make_float3(s_res[idx], s_res[idx + kSSWave * kSSChunk], s_res[idx + 2 * kSSWave * kSSChunk]);
         const float4 w = s_w[i0 + c];
         const float br = s_bray[i0 + c];
         const float3 t3 = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
@@ -438,12 +483,37 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
 // entry of the ray, so the workgroup first folds those two dimensions into a (sun elevation,
 // heading) slab in LDS (coalesced reads of four dJ slabs), Q samples at a time; each lane then
 // interpolates its own sun-elevation / heading coordinates in that slab.
+// One entry's samples i0 .. i0 + qn - 1 of the folded slabs: sun-elevation coordinate from
+// dot(p_i, l) / |p_i|, heading lerp La fixed along the ray.
+template <bool kColumn>
+__device__ __forceinline__ float3 ray_scatter_samples(const DevParams& P, const float4* s_slab, const float4* s_w,
+                                                      const float2* s_g, const float* s_rinv, int i0, int qn, int NL,
+                                                      int NA, int NLA, float vl, float xl, Lerp La, float3 acc) {
+  for (int q = 0; q < qn; ++q) {
+    const int i = i0 + q;
+    const float4 w = s_w[i];
+    const float sin_e = fmaf(s_g[i].y, vl, xl) * s_rinv[i];
+    const Lerp Ls = make_lerp(sun_elevation_coord(P, NL, sin_e), NL);
+    float3 v;
+    if (kColumn) {
+      const float4* col = s_slab + q * NLA + La.i0;
+      v = mix3(ld3(col, Ls.i0 * NA), ld3(col, Ls.i1 * NA), Ls.s);
+    } else {
+      v = lookup2(s_slab + q * NLA, NA, Ls, La);
+    }
+    acc.x += w.x * v.x; acc.y += w.y * v.y; acc.z += w.z * v.z;
+  }
+  return acc;
+}
+
 template <int EPT>
 __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
                                                              float4* __restrict__ out, int Q, WorkCounters* wc) {
   __shared__ float4 s_w[kMaxSteps];  // T(x, p_i) rgb, w: height coordinate of p_i
   __shared__ float2 s_g[kMaxSteps];  // x: elevation coordinate of v at p_i, y: t_i
   __shared__ float s_rinv[kMaxSteps];
+  __shared__ int4 s_off[kMaxRayChunk];   // slab offsets (in units of NL x NA) of the chunk's samples
+  __shared__ float2 s_wt[kMaxRayChunk];  // elevation and height weights of the chunk's samples
   extern __shared__ float4 s_slab[];  // Q x (NL x NA)
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
@@ -482,31 +552,37 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
     }
   }
   __syncthreads();
-  const long long sh = (long long)NE * NLA;  // stride of one height slab
   for (int i0 = 0; i0 < n; i0 += Q) {
     const int qn = min(Q, n - i0);
-    for (int idx = threadIdx.x; idx < qn * NLA; idx += blockDim.x) {
-      int q = idx / NLA, kk = idx - q * NLA;
-      int i = i0 + q;
-      Lerp Lh = make_lerp(s_w[i].w, NH), Le = make_lerp(s_g[i].x, NE);
-      const float4* b0 = dJ + Lh.i0 * sh + kk;
-      const float4* b1 = dJ + Lh.i1 * sh + kk;
-      float3 u = mix3(ld3(b0, (long long)Le.i0 * NLA), ld3(b0, (long long)Le.i1 * NLA), Le.s);
-      float3 v = mix3(ld3(b1, (long long)Le.i0 * NLA), ld3(b1, (long long)Le.i1 * NLA), Le.s);
-      float3 c = mix3(u, v, Lh.s);
-      s_slab[idx] = make_float4(c.x, c.y, c.z, 0.f);
+    if (threadIdx.x < qn) {
+      // element offsets of the four (height, elevation) slabs of sample i0 + q and their two weights,
+      // shared by the workgroup
+      const int i = i0 + threadIdx.x;
+      const Lerp Lh = make_lerp(s_w[i].w, NH), Le = make_lerp(s_g[i].x, NE);
+      s_off[threadIdx.x] = make_int4((Lh.i0 * NE + Le.i0) * NLA, (Lh.i0 * NE + Le.i1) * NLA,
+                                     (Lh.i1 * NE + Le.i0) * NLA, (Lh.i1 * NE + Le.i1) * NLA);
+      s_wt[threadIdx.x] = make_float2(Le.s, Lh.s);
+    }
+    __syncthreads();
+    for (int q = 0; q < qn; ++q) {
+      const int4 o = s_off[q];
+      const float2 w = s_wt[q];
+      for (int kk = threadIdx.x; kk < NLA; kk += blockDim.x) {
+        const float3 u = mix3(ld3(dJ, (unsigned)(o.x + kk)), ld3(dJ, (unsigned)(o.y + kk)), w.x);
+        const float3 v = mix3(ld3(dJ, (unsigned)(o.z + kk)), ld3(dJ, (unsigned)(o.w + kk)), w.x);
+        const float3 c = mix3(u, v, w.y);
+        s_slab[q * NLA + kk] = make_float4(c.x, c.y, c.z, 0.f);
+      }
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       if (threadIdx.x + e * blockDim.x < NLA) {
-        for (int q = 0; q < qn; ++q) {
-          int i = i0 + q;
-          float4 w = s_w[i];
-          float sin_e = fmaf(s_g[i].y, vl[e], xl[e]) * s_rinv[i];
-          float3 v = lookup2(s_slab + q * NLA, NA, make_lerp(sun_elevation_coord(P, NL, sin_e), NL), La[e]);
-          acc[e].x += w.x * v.x; acc[e].y += w.y * v.y; acc[e].z += w.z * v.z;
-        }
+        // a heading coordinate with zero weight (every entry but the clamped ones) reads one column
+        if (La[e].s == 0.0f)
+          acc[e] = ray_scatter_samples<true>(P, s_slab, s_w, s_g, s_rinv, i0, qn, NL, NA, NLA, vl[e], xl[e], La[e], acc[e]);
+        else
+          acc[e] = ray_scatter_samples<false>(P, s_slab, s_w, s_g, s_rinv, i0, qn, NL, NA, NLA, vl[e], xl[e], La[e], acc[e]);
       }
     }
     __syncthreads();
@@ -612,6 +688,48 @@ struct DirLds {
   float4 phat;    // P / |P|
 };
 
+// Sum over the quadrature directions of one entry (sun direction l, sun-elevation lerp Ls).
+template <bool kRow>
+__device__ __forceinline__ float3 point_scatter_dirs(const DevParams& P, const DirLds* s_dir, int ndirs,
+                                                     const float4* __restrict__ sh, const float4* __restrict__ shm,
+                                                     const float4* __restrict__ dE, int NA, int NLA, Lerp Ls, float lx,
+                                                     float ly, float lz) {
+  float3 acc = make_float3(0.f, 0.f, 0.f);
+  for (int j = 0; j < ndirs; ++j) {
+    const DirLds& dd = s_dir[j];
+    const float ol = dd.omega.x * lx + dd.omega.y * ly + dd.omega.z * lz;
+    const Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
+    const float4* row = sh + (long long)j * NLA;
+    float3 v;
+    if (kRow) {
+      row += Ls.i0 * NA;
+      v = mix3(ld3(row, La.i0), ld3(row, La.i1), La.s);
+    } else {
+      v = lookup2(row, NA, Ls, La);
+    }
+    if (shm) {
+      const float4* rowm = shm + (long long)j * NLA;
+      float3 m;
+      if (kRow) {
+        rowm += Ls.i0 * NA;
+        m = mix3(ld3(rowm, La.i0), ld3(rowm, La.i1), La.s);
+      } else {
+        m = lookup2(rowm, NA, Ls, La);
+      }
+      const float ph = phase_f(P.g_mie, ol);
+      v.x += m.x * ph; v.y += m.y * ph; v.z += m.z * ph;
+    }
+    if (dd.omega.w != 0.f) {
+      const float spl = dd.phat.x * lx + dd.phat.y * ly + dd.phat.z * lz;
+      const float3 e = lookup2(dE, P.nE[1], make_lerp(dd.ground.w, P.nE[0]),
+                               make_lerp(sun_elevation_coord(P, P.nE[1], spl), P.nE[1]));
+      v.x += dd.ground.x * e.x; v.y += dd.ground.y * e.y; v.z += dd.ground.z * e.z;
+    }
+    acc.x += dd.weight.x * v.x; acc.y += dd.weight.y * v.y; acc.z += dd.weight.z * v.z;
+  }
+  return acc;
+}
+
 __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
                                                                const DirGeom* __restrict__ geom, int ndirs,
                                                                const float4* __restrict__ slab,
@@ -654,27 +772,13 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
     index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2d);
     const float lx = (float)l0, ly = (float)l1, lz = (float)l2d;
     const Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
-    float3 acc = make_float3(0.f, 0.f, 0.f);
-    for (int j = 0; j < ndirs; ++j) {
-      const DirLds& dd = s_dir[j];
-      float ol = dd.omega.x * lx + dd.omega.y * ly + dd.omega.z * lz;
-      Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
-      float3 v = lookup2(sh + (long long)j * NLA, NA, Ls, La);
-      if (shm) {
-        float3 m = lookup2(shm + (long long)j * NLA, NA, Ls, La);
-        float ph = phase_f(P.g_mie, ol);
-        v.x += m.x * ph; v.y += m.y * ph; v.z += m.z * ph;
-      }
-      if (dd.omega.w != 0.f) {
-        float spl = dd.phat.x * lx + dd.phat.y * ly + dd.phat.z * lz;
-        float3 e = lookup2(dE, P.nE[1], make_lerp(dd.ground.w, P.nE[0]),
-                           make_lerp(sun_elevation_coord(P, P.nE[1], spl), P.nE[1]));
-        v.x += dd.ground.x * e.x; v.y += dd.ground.y * e.y; v.z += dd.ground.z * e.z;
-        ++l2;
-      }
-      acc.x += dd.weight.x * v.x; acc.y += dd.weight.y * v.y; acc.z += dd.weight.z * v.z;
-    }
+    // a grid entry's own sun-elevation coordinate is an integer, so its weight is exactly zero and only
+    // one row of each slab is read (lookup2_skip, bit for bit); the general lookup stays for safety
+    const float3 acc = Ls.s == 0.0f ? point_scatter_dirs<true>(P, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, lx, ly, lz)
+                                    : point_scatter_dirs<false>(P, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, lx, ly, lz);
     l4 += (unsigned long long)ndirs * (shm ? 2 : 1);
+    if (wc)
+      for (int j = 0; j < ndirs; ++j) l2 += s_dir[j].omega.w != 0.f;
     out[base + k] = make_float4(acc.x, acc.y, acc.z, 0.f);
   }
   count(wc, 0, 0, l4, l2);
@@ -857,6 +961,11 @@ __global__ void residual_kernel(const float4* __restrict__ a, const float4* __re
   if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(r));
 }
 
+__global__ void fill_zero_kernel(float* __restrict__ p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
 inline unsigned blocks_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -879,7 +988,7 @@ hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* m
   unsigned groups = (unsigned)((P.nS[2] * P.nS[3] + kSSWave - 1) / kSSWave);
   int n = P.ray_steps;
   size_t shmem = (size_t)n * (2 * sizeof(double) + sizeof(float4)) + sizeof(float) * (size_t)((n + 3) & ~3) +
-                 (sizeof(float4) + sizeof(unsigned short)) * kSSWave * kSSChunk + 2 * sizeof(double) * kSSWave;
+                 (3 * sizeof(float) + sizeof(unsigned short)) * kSSWave * kSSChunk + 2 * sizeof(double) * kSSWave;
   hipLaunchKernelGGL(single_scatter_kernel, dim3(rays * groups), dim3(kSSWave), shmem, stream, P, rayleigh, mie, wc);
   return hipGetLastError();
 }
@@ -964,6 +1073,12 @@ hipError_t launch_residual(const float4* a, const float4* x, const float4* y, lo
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, x, y, count, scratch4,
                      fraction, scratch4 + 3);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(float* p, long long n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, p, n);
   return hipGetLastError();
 }
 

@@ -39,6 +39,8 @@ hipError_t launch_add4(const float4* a, const float4* b, float4* c, long long co
 hipError_t launch_residual(const float4* a, const float4* x, const float4* y, long long count, float fraction,
                            unsigned* scratch4, hipStream_t stream);
 // Pack a float4 grid-order table into RGB floats (grid order), for the stage entry points.
+// p[0 .. n) = 0 (a kernel, so that graph captures hold no memset node).
+hipError_t launch_fill_zero(float* p, long long n, hipStream_t stream);
 hipError_t launch_pack_rgb(const float4* in, float* out, long long count, hipStream_t stream);
 hipError_t launch_unpack_rgb(const float* in, float4* out, long long count, hipStream_t stream);
 

@@ -155,14 +155,37 @@ def test_generate_luts_small(atm, gpu, oracle, iterations, brightness):
     assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
 
 
+TABLES = ["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]
+
+
 def test_build_fetch_matches_generate(atm, gpu):
     P = small_params(atm, iterations=2)
     ref = gpu.generate_atmosphere_luts(P)
-    gpu.build(P)
-    for which, name in enumerate(["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]):
+    gpu.build_timed(P)
+    for which, name in enumerate(TABLES):
         assert np.array_equal(gpu.fetch(P, which), ref[name]), name
     timings = gpu.stage_timings()
     assert timings["single_scatter"][1] == 1 and timings["point_scatter"][1] == 2
+
+
+@pytest.mark.parametrize("iterations", [0, 2])
+def test_graph_replays_equal_eager_builds(atm, gpu, iterations):
+    """sfsim_atmo_build replays a captured graph: every replay, with eager work and other parameters in
+    between, equals the eager kernel-by-kernel build bit for bit (iterations 0 covers the zero-filled
+    surface-radiance table)."""
+    P = small_params(atm, iterations=iterations)
+    Q = small_params(atm, iterations=1, brightness=0.1)
+    eager = gpu.generate_atmosphere_luts(P)
+    other = gpu.generate_atmosphere_luts(Q)
+    for rep in range(4):
+        gpu.build(P)
+        for which, name in enumerate(TABLES):
+            assert np.array_equal(gpu.fetch(P, which), eager[name]), (rep, name)
+        gpu.transmittance_table(Q)  # eager work between replays
+        if rep == 2:  # a different parameter set: re-capture, then back
+            gpu.build(Q)
+            for which, name in enumerate(TABLES):
+                assert np.array_equal(gpu.fetch(Q, which), other[name]), name
 
 
 def test_determinism(atm, gpu):
