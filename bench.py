@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-seeded", action="store_true", help="skip the surrogate-seeded leg")
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(16, cores))")
     return ap.parse_args()
 
@@ -114,6 +116,33 @@ def cpu_baseline(P, threads):
     return total, sampled_entries, sampled_seconds
 
 
+def seeded_leg(atm, ctx, P, workload, steps, train_steps):
+    """The surrogate-seeded 5-order build next to the plain one: the network is trained on the GPU on
+    the plain iteration's order sums of this atmosphere and 7 neighbours drawn from the training ranges
+    (training time reported separately), then seeded builds with m = 1 and m = 2 exact applications are
+    timed and checked against the same FP64-oracle fixture.  Tolerance infinity: the residual is
+    reported, not acted on, so the timed work is the seeded path itself."""
+    rng = np.random.default_rng(20261202)
+    family = [P] + [atm.random_atmosphere(rng, P) for _ in range(7)]
+    sg = atm.Surrogate(ctx, seed=1, batch=8192)
+    t0 = time.perf_counter()
+    losses = sg.train(family, steps=train_steps, lr=2e-3, seed=2)
+    train_s = time.perf_counter() - t0
+    out = {"train": {"atmospheres": len(family), "steps": train_steps, "batch": 8192, "seconds": round(train_s, 2),
+                     "loss_first": float(losses[0]), "loss_last": float(np.mean(losses[-20:]))}}
+    n_entries = atm.entries(P)
+    for m in (1, 2):
+        atm.build_seeded(ctx, P, m, tolerance=float("inf"))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r, _ = atm.build_seeded(ctx, P, m, tolerance=float("inf"))
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        out[f"m{m}"] = {"ms_per_step": round(ms, 4), "value": round(n_entries / (ms / 1e3), 1), "residual": r,
+                        "falls_back_at_1e-3": bool(r > 1e-3),
+                        "rel_err_vs_oracle": check_against_fixture(atm, ctx, P, workload)}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -157,6 +186,12 @@ def main():
                 "avg_launch_ms": round(ss_avg_ms, 4), "algorithmic_flops": ss_flops,
                 "work": work}
 
+    seeded = None
+    if not args.no_seeded and P.iterations == 5:
+        seeded = seeded_leg(atm, ctx, P, args.workload, args.steps, args.train_steps)
+        for m in ("m1", "m2"):
+            seeded[m]["speedup_vs_plain"] = round(ms_per_step / seeded[m]["ms_per_step"], 3)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -177,6 +212,7 @@ def main():
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kernel_ms.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "surrogate_seeded": seeded,
     }
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -158,8 +158,9 @@ int sfsim_atmo_surrogate_set_weights(sfsim_atmo_ctx* ctx, const float* in, size_
 int sfsim_atmo_surrogate_eval(sfsim_atmo_ctx* ctx, const float* inputs, const float* targets, int rows,
                               float* outputs, float* grad, float* loss);
 /* Trains on the 5-order plain builds of `count` atmospheres sharing one scatter shape: `steps` Adam
- * steps of one batch each, learning rate lr with cosine decay; losses (nullable) receives one value per
- * step. */
+ * steps of one batch each, learning rate lr with a linear warm-up over the first 5% of the steps and
+ * cosine decay after it (a fresh network's output biases start at the targets' means); losses
+ * (nullable) receives one value per step. */
 int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atmospheres, int count, int steps,
                                double lr, unsigned long long seed, float* losses);
 /* Device-resident 5-order build seeded by the network with `applications` (1..4) exact applications

@@ -1038,11 +1038,17 @@ int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atm
                               ctx->stream));
   const DevParams P = make_dev_params(&p0);
   Surrogate& sg = ctx->sg;
+  // a fresh network (no Adam step taken yet) starts from the targets' means
+  if (sg.adam_step == 0)
+    HIP_TRY(ctx, surrogate_init_output_bias(sg, (const float*)ctx->targets.ptr, (long long)count * (long long)nS,
+                                            ctx->stream));
   const int rows = sg.batch_capacity;
   const int parts = (rows * kMlpOut + 255) / 256;
   std::vector<float> part(parts);
+  const int warmup = steps / 20;  // linear warm-up over 5% of the steps, then cosine decay
   for (int step = 0; step < steps; ++step) {
-    float lr_t = (float)(lr * 0.5 * (1.0 + std::cos(M_PI * step / steps)));
+    float lr_t = step < warmup ? (float)(lr * (step + 1) / warmup)
+                               : (float)(lr * 0.5 * (1.0 + std::cos(M_PI * (step - warmup) / (steps - warmup))));
     HIP_TRY(ctx, surrogate_gather_batch(sg, P, (const AtmosphereCode*)ctx->codes.ptr, count,
                                         (const float*)ctx->targets.ptr, seed, step, rows, ctx->stream));
     HIP_TRY(ctx, surrogate_forward(sg, rows, ctx->stream));

@@ -188,6 +188,24 @@ __global__ void dsilu_kernel(const float* __restrict__ dA, const float* __restri
   dZ[i] = f2bf(g);
 }
 
+// out[c] = mean over rows of T[row][c] for the kMlpUsed target columns (row stride kMlpUsed), FP64
+// partial sums in a fixed order: the output biases' initial values.
+__global__ void __launch_bounds__(kEw) target_mean_kernel(const float* __restrict__ T, long long rows,
+                                                          float* __restrict__ out) {
+  __shared__ double red[kEw / 64];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (long long r = threadIdx.x; r < rows; r += blockDim.x) s += T[r * kMlpUsed + c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kEw / 64; ++w) t += red[w];
+    out[c] = (float)(t / (double)rows);
+  }
+}
+
 // out[c] = sum over rows of G[row][c] (one workgroup per column, fixed summation order).
 __global__ void __launch_bounds__(kEw) colsum_kernel(const float* __restrict__ G, int rows, int cols,
                                                      float* __restrict__ out) {
@@ -564,6 +582,12 @@ hipError_t surrogate_gather_batch(Surrogate& s, const DevParams& P, const Atmosp
                                   const float* targets_dev, uint64_t seed, int step, int rows, hipStream_t stream) {
   hipLaunchKernelGGL(gather_kernel, dim3(ew_blocks(rows)), dim3(kEw), 0, stream, P, codes_dev, natm, targets_dev, seed,
                      step, rows, s.X, s.T);
+  return hipGetLastError();
+}
+
+hipError_t surrogate_init_output_bias(Surrogate& s, const float* targets_dev, long long rows, hipStream_t stream) {
+  hipLaunchKernelGGL(target_mean_kernel, dim3(kMlpUsed), dim3(kEw), 0, stream, targets_dev, rows,
+                     s.theta + MlpLayout::b4);
   return hipGetLastError();
 }
 

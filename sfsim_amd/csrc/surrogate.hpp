@@ -96,6 +96,8 @@ hipError_t surrogate_adam(Surrogate& s, float lr, hipStream_t stream);
 // `entries` scatter-grid entries each; targets[atm][entry][9] hold log(P + eps) + shift.
 hipError_t surrogate_gather_batch(Surrogate& s, const DevParams& P, const AtmosphereCode* codes_dev, int natm,
                                   const float* targets_dev, uint64_t seed, int step, int rows, hipStream_t stream);
+// Output biases b4[0 .. kMlpUsed) = mean of each target column over `rows` target rows.
+hipError_t surrogate_init_output_bias(Surrogate& s, const float* targets_dev, long long rows, hipStream_t stream);
 // Targets of one atmosphere from its order sums P_1..P_5 (float4 tables).
 hipError_t surrogate_make_targets(const float4* const* sums, long long entries, float* targets_dev, hipStream_t stream);
 // Predicts the order sums P_k for every entry of the scatter grid of one atmosphere into out[k - 1]
