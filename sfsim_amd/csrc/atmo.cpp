@@ -552,7 +552,9 @@ AtmosphereCode code_of(const sfsim_atmo_params* p) {
 // (dS_0 + P_j, E0 + E[dS_0 + P_{j-1}]).  Seeding X_{5-m} with the network's P_{5-m}, P_{4-m} and applying
 // the recurrence m times gives X_5 with the network's error multiplied by L^m (a contraction);
 // m = `applications` in 1..4.  The network's own P_5 against the computed sum is the residual; above
-// `tol` the plain iteration runs instead.
+// `tol` the plain iteration runs instead.  The final ray-scatter table then follows the plain
+// iteration's accumulation, re-sampled order by order, on P_1..P_j of the seed and the computed
+// P_{j+1}..P_5.
 // host_sums (nullable): the order sums P_1..P_5 given by the caller (5 RGB scatter tables) instead of
 // the network's.
 int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double tol, double* residual, int* fallback,
@@ -581,7 +583,7 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
   TRY(ensure(ctx, ctx->M1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->dJ, nS * sizeof(float4)));
   for (int i = 0; i < kMlpOrders; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
-  TRY(ensure(ctx, ctx->resid, 4 * sizeof(unsigned)));
+  TRY(ensure(ctx, ctx->resid, residual_scratch_floats((long long)nS) * sizeof(float)));
   if (host_sums) TRY(ensure(ctx, ctx->stage_rgb, nS * 3 * sizeof(float)));
   const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
   for (int i = 0; i < 4; ++i) {
@@ -601,18 +603,17 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
   TRY(timer.begin(kDirectionGeometry));
   TRY(prepare_point_scatter(ctx, P, true, nullptr));
   TRY(timer.end());
-  // network: P_j, P_{j-1} (seed) and P_5 (residual)
-  float4* want[kMlpOrders] = {};
-  want[j - 1] = f4(ctx->pred[j - 1]);
-  if (j >= 2) want[j - 2] = f4(ctx->pred[j - 2]);
-  want[kMlpOrders - 1] = f4(ctx->pred[kMlpOrders - 1]);
+  // the network's order sums: P_1..P_4 into pred[0..3] (the ones below the seed feed the final
+  // table's re-sampling chain), its P_5 into dS[1] (residual)
+  float4* want[kMlpOrders];
+  for (int k = 0; k + 1 < kMlpOrders; ++k) want[k] = f4(ctx->pred[k]);
+  want[kMlpOrders - 1] = f4(ctx->dS[1]);
   if (host_sums) {
-    for (int k = 0; k < kMlpOrders; ++k)
-      if (want[k]) {
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->stage_rgb.ptr, host_sums + (size_t)k * nS * 3, nS * 3 * sizeof(float),
-                                    hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, launch_unpack_rgb((const float*)ctx->stage_rgb.ptr, want[k], n4, ctx->stream));
-      }
+    for (int k = 0; k < kMlpOrders; ++k) {
+      HIP_TRY(ctx, hipMemcpyAsync(ctx->stage_rgb.ptr, host_sums + (size_t)k * nS * 3, nS * 3 * sizeof(float),
+                                  hipMemcpyHostToDevice, ctx->stream));
+      HIP_TRY(ctx, launch_unpack_rgb((const float*)ctx->stage_rgb.ptr, want[k], n4, ctx->stream));
+    }
   } else {
     TRY(timer.begin(kSurrogate));
     HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), want, ctx->stream));
@@ -623,9 +624,9 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
   float4* A = f4(ctx->Sacc[0]);
   float4* B = f4(ctx->Sacc[1]);
   float4* Esum = f4(ctx->Eacc[0]);
-  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), want[j - 1], A, n4, ctx->stream));
+  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->pred[j - 1]), A, n4, ctx->stream));
   if (j >= 2) {
-    LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), want[j - 2], B, n4, ctx->stream));
+    LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->pred[j - 2]), B, n4, ctx->stream));
     LAUNCH(timer, kSurfaceRadiance,
            launch_surface_radiance(P, hdirs, ctx->half_n, B, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
   } else {
@@ -633,29 +634,30 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
                                                             f4(ctx->dE[1]), nullptr, ctx->stream));
   }
   LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->dE[0]), f4(ctx->dE[1]), Esum, e4, ctx->stream));
-  // m applications of X <- x_0 + L X; after the last, dS[0] = P_5 and dE[1] = dE_1 + ... + dE_5
+  // m applications of X <- x_0 + L X: application a leaves the computed P_{j+1+a} in pred[j + a]; after
+  // the last, dE[1] = dE_1 + ... + dE_5
   for (int a = 0; a < m; ++a) {
+    float4* Pk = f4(ctx->pred[j + a]);
     LAUNCH(timer, kPointScatter,
            launch_point_scatter(P, sdirs, ctx->sphere_n, ctx->geom.ptr, A, f4(ctx->M1), Esum, f4(ctx->slab),
                                 f4(ctx->slab_mie), f4(ctx->dJ), nullptr, ctx->stream));
-    LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[0]), nullptr, ctx->stream));
+    LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), Pk, nullptr, ctx->stream));
     LAUNCH(timer, kSurfaceRadiance,
            launch_surface_radiance(P, hdirs, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
     if (a + 1 < m) {
-      LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->dS[0]), A, n4, ctx->stream));
+      LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), Pk, A, n4, ctx->stream));
       LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->dE[0]), f4(ctx->dE[1]), Esum, e4, ctx->stream));
     }
   }
-  // residual of the network's own P_5, relative to R1 + P_5
+  // residual of the network's own P_5 against the computed one, relative to R1 + P_5
   TRY(timer.begin(kSurrogate));
-  HIP_TRY(ctx, launch_residual(f4(ctx->R1), f4(ctx->dS[0]), want[kMlpOrders - 1], n4, 1e-6f, (unsigned*)ctx->resid.ptr,
-                               ctx->stream));
+  HIP_TRY(ctx, launch_residual(f4(ctx->R1), f4(ctx->pred[kMlpOrders - 1]), f4(ctx->dS[1]), n4, 1e-6f,
+                               (float*)ctx->resid.ptr, ctx->stream));
   TRY(timer.end());
-  unsigned bits[4];
-  HIP_TRY(ctx, hipMemcpyAsync(bits, ctx->resid.ptr, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+  float rr[8];
+  HIP_TRY(ctx, hipMemcpyAsync(rr, ctx->resid.ptr, sizeof(rr), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  float r;
-  std::memcpy(&r, &bits[3], sizeof(r));
+  const float r = rr[4];
   *residual = r;
   ctx->last_residual = r;
   if (!(r <= tol)) {
@@ -663,12 +665,25 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
     return run_pipeline(ctx, p, nullptr, true);
   }
   *fallback = 0;
-  LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->dS[0]), f4(ctx->dS[1]), n4, ctx->stream));
+  // the plain iteration's accumulation S_k = tab(S_{k-1} + dS_k), S_0 = R1 (atmosphere_lut.clj:95-97), on
+  // the orders dS_k = P_k - P_{k-1}: re-sampling at forward(backward(i)) is not the identity where the
+  // sun direction is clamped, so the order-by-order chain is what the plain tables hold there
+  const float4* S_cur = f4(ctx->R1);
+  int s_next = 0;
+  for (int k = 1; k <= kMlpOrders; ++k) {
+    const float4* dk = f4(ctx->pred[0]);
+    if (k > 1) {
+      LAUNCH(timer, kAccumulate, launch_sub4(f4(ctx->pred[k - 1]), f4(ctx->pred[k - 2]), f4(ctx->dJ), n4, ctx->stream));
+      dk = f4(ctx->dJ);
+    }
+    LAUNCH(timer, kAccumulate, launch_resample(P, 4, S_cur, dk, f4(ctx->Sacc[s_next]), nullptr, 0, ctx->stream));
+    S_cur = f4(ctx->Sacc[s_next]);
+    s_next = 1 - s_next;
+  }
   LAUNCH(timer, kFinal, launch_resample(P, 2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, ctx->stream));
   LAUNCH(timer, kFinal,
          launch_resample(P, 3, f4(ctx->dE[1]), nullptr, nullptr, (float*)ctx->out[1].ptr, 0, ctx->stream));
-  LAUNCH(timer, kFinal,
-         launch_resample(P, 4, f4(ctx->dS[1]), nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
+  LAUNCH(timer, kFinal, launch_resample(P, 4, S_cur, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
   LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   TRY(timer.collect());

@@ -922,43 +922,75 @@ __global__ void add4_kernel(const float4* __restrict__ a, const float4* __restri
   c[i] = make_float4(x.x + y.x, x.y + y.y, x.z + y.z, 0.f);
 }
 
-// Non-negative float maxima as ordered unsigned integers (atomicMax on the bit patterns).
-__global__ void channel_max_kernel(const float4* __restrict__ a, const float4* __restrict__ b, long long count,
-                                   unsigned* __restrict__ out) {
+__global__ void sub4_kernel(const float4* __restrict__ a, const float4* __restrict__ b, float4* __restrict__ c,
+                            long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float4 x = a[i], y = b[i];
+  c[i] = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, 0.f);
+}
+
+// Block maxima of |a + x| per channel: partial[block] (x, y, z).  Two passes without atomics (the
+// second is one block over the partials), deterministic and free of memsets.
+__device__ __forceinline__ float3 block_max3(float3 m, float3* red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    m.x = fmaxf(m.x, __shfl_down(m.x, off, 64));
+    m.y = fmaxf(m.y, __shfl_down(m.y, off, 64));
+    m.z = fmaxf(m.z, __shfl_down(m.z, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      m = make_float3(fmaxf(m.x, red[w].x), fmaxf(m.y, red[w].y), fmaxf(m.z, red[w].z));
+  return m;  // valid in thread 0
+}
+
+__global__ void __launch_bounds__(kBlock) channel_max_kernel(const float4* __restrict__ a, const float4* __restrict__ b,
+                                                             long long count, float4* __restrict__ partial) {
+  __shared__ float3 red[kBlock / 64];
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   float3 m = make_float3(0.f, 0.f, 0.f);
   if (i < count) {
     float4 x = a[i], y = b[i];
     m = make_float3(fabsf(x.x + y.x), fabsf(x.y + y.y), fabsf(x.z + y.z));
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    m.x = fmaxf(m.x, __shfl_down(m.x, off, 64));
-    m.y = fmaxf(m.y, __shfl_down(m.y, off, 64));
-    m.z = fmaxf(m.z, __shfl_down(m.z, off, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(out + 0, __float_as_uint(m.x));
-    atomicMax(out + 1, __float_as_uint(m.y));
-    atomicMax(out + 2, __float_as_uint(m.z));
-  }
+  m = block_max3(m, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_float4(m.x, m.y, m.z, 0.f);
 }
 
-// max over entries and channels of |x - y| / max(|a + x|, floor_c), floor_c = fraction * channel max.
-__global__ void residual_kernel(const float4* __restrict__ a, const float4* __restrict__ x, const float4* __restrict__ y,
-                                long long count, const unsigned* __restrict__ cmax, float fraction,
-                                unsigned* __restrict__ out) {
+// out[0] = maximum over the n partials (one block).
+__global__ void __launch_bounds__(kBlock) max_partials_kernel(const float4* __restrict__ partial, int n,
+                                                              float4* __restrict__ out) {
+  __shared__ float3 red[kBlock / 64];
+  float3 m = make_float3(0.f, 0.f, 0.f);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    float4 v = partial[i];
+    m = make_float3(fmaxf(m.x, v.x), fmaxf(m.y, v.y), fmaxf(m.z, v.z));
+  }
+  m = block_max3(m, red);
+  if (threadIdx.x == 0) out[0] = make_float4(m.x, m.y, m.z, 0.f);
+}
+
+// Block maxima over entries and channels of |x - y| / max(|a + x|, floor_c), floor_c = fraction *
+// channel max (cmax), in partial[block].x.
+__global__ void __launch_bounds__(kBlock) residual_kernel(const float4* __restrict__ a, const float4* __restrict__ x,
+                                                          const float4* __restrict__ y, long long count,
+                                                          const float4* __restrict__ cmax, float fraction,
+                                                          float4* __restrict__ partial) {
+  __shared__ float3 red[kBlock / 64];
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   float r = 0.f;
   if (i < count) {
+    const float4 c = cmax[0];
     float4 av = a[i], xv = x[i], yv = y[i];
-    float f0 = fraction * __uint_as_float(cmax[0]), f1 = fraction * __uint_as_float(cmax[1]),
-          f2 = fraction * __uint_as_float(cmax[2]);
+    float f0 = fraction * c.x, f1 = fraction * c.y, f2 = fraction * c.z;
     r = fmaxf(fabsf(xv.x - yv.x) / fmaxf(fabsf(av.x + xv.x), fmaxf(f0, 1e-30f)),
               fmaxf(fabsf(xv.y - yv.y) / fmaxf(fabsf(av.y + xv.y), fmaxf(f1, 1e-30f)),
                     fabsf(xv.z - yv.z) / fmaxf(fabsf(av.z + xv.z), fmaxf(f2, 1e-30f))));
   }
-  for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_down(r, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(r));
+  float3 m = block_max3(make_float3(r, r, r), red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_float4(m.x, 0.f, 0.f, 0.f);
 }
 
 __global__ void fill_zero_kernel(float* __restrict__ p, long long n) {
@@ -1059,20 +1091,32 @@ hipError_t launch_resample(const DevParams& P, int space, const float4* a, const
   return hipGetLastError();
 }
 
+hipError_t launch_sub4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream) {
+  hipLaunchKernelGGL(sub4_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, b, c, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_add4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream) {
   hipLaunchKernelGGL(add4_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, b, c, count);
   return hipGetLastError();
 }
 
+size_t residual_scratch_floats(long long count) { return 4 * (2 + (size_t)blocks_for(count)); }
+
 hipError_t launch_residual(const float4* a, const float4* x, const float4* y, long long count, float fraction,
-                           unsigned* scratch4, hipStream_t stream) {
-  hipError_t e = hipMemsetAsync(scratch4, 0, 4 * sizeof(unsigned), stream);
+                           float* scratch, hipStream_t stream) {
+  const unsigned nb = blocks_for(count);
+  float4* s4 = (float4*)scratch;  // [0] channel maxima, [1] result, [2 ..] block partials
+  hipLaunchKernelGGL(channel_max_kernel, dim3(nb), dim3(kBlock), 0, stream, a, x, count, s4 + 2);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(channel_max_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, x, count, scratch4);
+  hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(kBlock), 0, stream, s4 + 2, (int)nb, s4);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(residual_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, x, y, count, scratch4,
-                     fraction, scratch4 + 3);
+  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(kBlock), 0, stream, a, x, y, count, s4, fraction, s4 + 2);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(kBlock), 0, stream, s4 + 2, (int)nb, s4 + 1);
   return hipGetLastError();
 }
 

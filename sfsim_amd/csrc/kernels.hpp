@@ -34,10 +34,14 @@ hipError_t launch_resample(const DevParams& P, int space, const float4* a, const
                            float* out_rgb, int tiled, hipStream_t stream);
 // c = a + b (float4 tables)
 hipError_t launch_add4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream);
-// scratch4[3] <- bits of max_i max_c |x - y| / max(|a + x|, fraction * max_i |a + x|_c) (scratch4[0..2]:
-// the channel maxima), all as float bit patterns.
+// c = a - b (RGB, w = 0)
+hipError_t launch_sub4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream);
+// scratch[4] <- max_i max_c |x - y| / max(|a + x|, fraction * max_i |a + x|_c) (scratch[0..2]: the
+// channel maxima); scratch holds residual_scratch_floats(count) floats.  Block reductions in two passes,
+// no atomics, no memset.
+size_t residual_scratch_floats(long long count);
 hipError_t launch_residual(const float4* a, const float4* x, const float4* y, long long count, float fraction,
-                           unsigned* scratch4, hipStream_t stream);
+                           float* scratch, hipStream_t stream);
 // Pack a float4 grid-order table into RGB floats (grid order), for the stage entry points.
 // p[0 .. n) = 0 (a kernel, so that graph captures hold no memset node).
 hipError_t launch_fill_zero(float* p, long long n, hipStream_t stream);
