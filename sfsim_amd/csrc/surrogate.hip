@@ -25,10 +25,14 @@ __host__ __device__ inline uint16_t f2bf(float f) {
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (1 ulp) instead of an IEEE division: the
+// results are rounded to bf16 for the products anyway.
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 
 __device__ __forceinline__ float dsilu(float z) {
-  float s = 1.0f / (1.0f + __expf(-z));
+  float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -274,42 +278,94 @@ __global__ void layer1_tables_kernel(const DevParams P, AtmosphereCode code, con
 }
 
 constexpr int kFusedWaves = 4;
-constexpr int kLdsRow = kMlpHidden + 4;  // padded row (bf16) of the transposed weights in LDS
+constexpr int kLdsRow = kMlpHidden + 8;  // padded row (bf16) of the transposed weights in LDS, 16-byte aligned
 constexpr int kFusedLds = (2 * kMlpHidden + 32) * kLdsRow * 2;
 
-__device__ __forceinline__ short4_t silu_bf16x4(float a, float b, float c, float d) {
-  short4_t r;
-  r[0] = (short)f2bf(silu(a));
-  r[1] = (short)f2bf(silu(b));
-  r[2] = (short)f2bf(silu(c));
-  r[3] = (short)f2bf(silu(d));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// Column order of the transposed weights in LDS: within each block of 16 input units the middle two
+// groups of 4 swap places, so that the 8 units a lane half supplies to one K = 16 step (see below) are
+// contiguous: one 16-byte LDS read per operand.
+__device__ __forceinline__ int lds_column(int k) {
+  const int g = (k >> 2) & 3;
+  const int gs = g == 1 ? 2 : (g == 2 ? 1 : g);
+  return (k & ~15) | (gs << 2) | (k & 3);
+}
+
+// bf16 (round to nearest even, v_cvt_pk_bf16_f32) of SiLU of 8 values.
+__device__ __forceinline__ bf16x8_t silu_bf16x8(float4 lo, float4 hi) {
+  bf16x8_t r;
+  r[0] = (__bf16)silu(lo.x);
+  r[1] = (__bf16)silu(lo.y);
+  r[2] = (__bf16)silu(lo.z);
+  r[3] = (__bf16)silu(lo.w);
+  r[4] = (__bf16)silu(hi.x);
+  r[5] = (__bf16)silu(hi.y);
+  r[6] = (__bf16)silu(hi.z);
+  r[7] = (__bf16)silu(hi.w);
   return r;
 }
 
+__device__ __forceinline__ float4 add5(float4 a, float4 b, float4 c, float4 d, float4 e) {
+  return make_float4(a.x + b.x + c.x + d.x + e.x, a.y + b.y + c.y + d.y + e.y, a.z + b.z + c.z + d.z + e.z,
+                     a.w + b.w + c.w + d.w + e.w);
+}
+
 // One wavefront evaluates 32 entries at a time with the hidden units as the row dimension of every
-// product (Z^T = W^T A^T): the accumulator of v_mfma_f32_32x32x8bf16_1k leaves lane l with hidden units
-// 8 g + 4 (l / 32) + i (g = 0..3, i = 0..3) of entry l % 32, which is exactly the lane's B operand of
-// the next layer's K-steps 8 g .. 8 g + 7, so activations never leave the registers.  W2^T, W3^T and
-// W4^T (padded to 32 rows) are staged once per workgroup in LDS; workgroups loop over the tiles.
+// product (Z^T = W^T A^T) on v_mfma_f32_32x32x16_bf16.  Its accumulator leaves lane l (half h = l / 32)
+// with hidden units 8 g + 4 h + i (g = 0..3, i = 0..3) of each block of 32, for entry l % 32.  A K = 16
+// step t over the previous layer's units 16 t .. 16 t + 15 takes from lane half h the 8 units
+// 16 t + 4 h + i and 16 t + 8 + 4 h + i (the same register positions in A and B pair up, whatever unit
+// they stand for), which that lane already holds: activations never leave the registers.  W2^T, W3^T and
+// W4^T (padded to 32 rows, columns in lds_column order) are staged once per workgroup in LDS;
+// workgroups loop over the tiles.
 struct PredTables {
   float4* p[kMlpOrders];
 };
 
-__global__ void __launch_bounds__(64 * kFusedWaves) fused_predict_kernel(
+__global__ void __launch_bounds__(64 * kFusedWaves, 2) fused_predict_kernel(
     const DevParams P, const uint16_t* __restrict__ wbf, const float* __restrict__ theta,
     const float* __restrict__ ztab, long long entries, const PredTables out) {
-  __shared__ __align__(16) uint16_t lds[kFusedLds / 2];
-  uint16_t* w2t = lds;
-  uint16_t* w3t = w2t + kMlpHidden * kLdsRow;
-  uint16_t* w4t = w3t + kMlpHidden * kLdsRow;
-  for (int i = threadIdx.x; i < kMlpHidden * kMlpHidden; i += blockDim.x) {
-    const int k = i / kMlpHidden, n = i % kMlpHidden;
-    w2t[n * kLdsRow + k] = wbf[MlpLayout::W2 + i];
-    w3t[n * kLdsRow + k] = wbf[MlpLayout::W3 + i];
-  }
-  for (int i = threadIdx.x; i < 32 * kMlpHidden; i += blockDim.x) {
-    const int n = i / kMlpHidden, k = i % kMlpHidden;
-    w4t[n * kLdsRow + k] = n < kMlpOut ? wbf[MlpLayout::W4 + k * kMlpOut + n] : (uint16_t)0;
+  __shared__ __align__(16) __bf16 lds[kFusedLds / 2];
+  __bf16* w2t = lds;
+  __bf16* w3t = w2t + kMlpHidden * kLdsRow;
+  __bf16* w4t = w3t + kMlpHidden * kLdsRow;
+  {
+    // 16-byte loads of 8 consecutive weights W[k][n .. n + 7] (all issued before the LDS scatter),
+    // transposed into W^T rows; W4 rows 16..31 of W4^T are zero
+    static_assert(MlpLayout::W2 % 8 == 0 && MlpLayout::W3 % 8 == 0 && MlpLayout::W4 % 8 == 0, "16-byte rows");
+    constexpr int kVec = kMlpHidden * kMlpHidden / 8 / (64 * kFusedWaves);  // loads per thread per matrix
+    uint4 v2[kVec], v3[kVec];
+    const uint4* s2 = (const uint4*)(wbf + MlpLayout::W2);
+    const uint4* s3 = (const uint4*)(wbf + MlpLayout::W3);
+    const uint4* s4 = (const uint4*)(wbf + MlpLayout::W4);
+#pragma unroll
+    for (int q = 0; q < kVec; ++q) {
+      v2[q] = s2[threadIdx.x + q * 64 * kFusedWaves];
+      v3[q] = s3[threadIdx.x + q * 64 * kFusedWaves];
+    }
+    const uint4 v4 = s4[threadIdx.x];  // 128 x 16 weights = 256 vectors of 8
+#pragma unroll
+    for (int q = 0; q < kVec; ++q) {
+      const int i = 8 * (threadIdx.x + q * 64 * kFusedWaves);
+      const int k = i / kMlpHidden, n = i % kMlpHidden, c = lds_column(k);
+      const uint16_t* a2 = (const uint16_t*)&v2[q];
+      const uint16_t* a3 = (const uint16_t*)&v3[q];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        w2t[(n + j) * kLdsRow + c] = __builtin_bit_cast(__bf16, a2[j]);
+        w3t[(n + j) * kLdsRow + c] = __builtin_bit_cast(__bf16, a3[j]);
+      }
+    }
+    {
+      const int i = 8 * threadIdx.x;
+      const int k = i / kMlpOut, n = i % kMlpOut, c = lds_column(k);
+      const uint16_t* a4 = (const uint16_t*)&v4;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) w4t[(n + j) * kLdsRow + c] = __builtin_bit_cast(__bf16, a4[j]);
+    }
+    for (int i = threadIdx.x; i < (32 - kMlpOut) * kMlpHidden; i += blockDim.x)
+      w4t[(kMlpOut + i / kMlpHidden) * kLdsRow + i % kMlpHidden] = __builtin_bit_cast(__bf16, (uint16_t)0);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -321,29 +377,27 @@ __global__ void __launch_bounds__(64 * kFusedWaves) fused_predict_kernel(
   const long long tiles = (entries + 31) / 32;
   for (long long t = (long long)blockIdx.x * kFusedWaves + wave; t < tiles; t += (long long)gridDim.x * kFusedWaves) {
     const long long e = t * 32 + r;
-    const long long ec = e < entries ? e : entries - 1;
-    const int ia = (int)(ec % NA);
-    long long q = ec / NA;
-    const int is = (int)(q % NL);
-    q /= NL;
-    const int ie = (int)(q % NE);
-    const int ih = (int)(q / NE);
+    const unsigned ec = (unsigned)(e < entries ? e : entries - 1);  // < 2^31: the host caps the grid
+    const int ia = (int)(ec % (unsigned)NA);
+    unsigned q = ec / (unsigned)NA;
+    const int is = (int)(q % (unsigned)NL);
+    q /= (unsigned)NL;
+    const int ie = (int)(q % (unsigned)NE);
+    const int ih = (int)(q / (unsigned)NE);
     const float4* z0 = (const float4*)ztab;
     const float4* zh = (const float4*)(ztab + (size_t)(1 + ih) * kMlpHidden);
     const float4* ze = (const float4*)(ztab + (size_t)(1 + NH + ie) * kMlpHidden);
     const float4* zs = (const float4*)(ztab + (size_t)(1 + NH + NE + is) * kMlpHidden);
     const float4* za = (const float4*)(ztab + (size_t)(1 + NH + NE + NL + ia) * kMlpHidden);
-    short4_t b[16];
+    bf16x8_t b[8];
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const int j4 = 2 * s + h;  // hidden units 8 s + 4 h .. + 3
-      const float4 a0 = z0[j4], a1 = zh[j4], a2 = ze[j4], a3 = zs[j4], a4 = za[j4];
-      b[s] = silu_bf16x4(a0.x + a1.x + a2.x + a3.x + a4.x, a0.y + a1.y + a2.y + a3.y + a4.y,
-                         a0.z + a1.z + a2.z + a3.z + a4.z, a0.w + a1.w + a2.w + a3.w + a4.w);
+    for (int s = 0; s < 8; ++s) {
+      const int lo = 4 * s + h, hi = 4 * s + 2 + h;  // float4 index of units 16 s + 4 h and 16 s + 8 + 4 h
+      b[s] = silu_bf16x8(add5(z0[lo], zh[lo], ze[lo], zs[lo], za[lo]), add5(z0[hi], zh[hi], ze[hi], zs[hi], za[hi]));
     }
 #pragma unroll 1
     for (int layer = 0; layer < 2; ++layer) {
-      const uint16_t* wt = layer ? w3t : w2t;
+      const __bf16* wt = layer ? w3t : w2t;
       const float* bias = layer ? b3 : b2;
       float16_t acc[4];
 #pragma unroll
@@ -351,28 +405,30 @@ __global__ void __launch_bounds__(64 * kFusedWaves) fused_predict_kernel(
 #pragma unroll
         for (int x = 0; x < 16; ++x) acc[mt][x] = 0.f;
 #pragma unroll
-      for (int s = 0; s < 16; ++s)
+      for (int s = 0; s < 8; ++s)
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
-          const short4_t a = *(const short4_t*)(wt + (32 * mt + r) * kLdsRow + 8 * s + 4 * h);
-          acc[mt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[s], acc[mt], 0, 0, 0);
+          const bf16x8_t a = *(const bf16x8_t*)(wt + (32 * mt + r) * kLdsRow + 16 * s + 8 * h);
+          acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], acc[mt], 0, 0, 0);
         }
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const float4 bb = *(const float4*)(bias + 32 * mt + 8 * g + 4 * h);
-          b[4 * mt + g] = silu_bf16x4(acc[mt][4 * g] + bb.x, acc[mt][4 * g + 1] + bb.y, acc[mt][4 * g + 2] + bb.z,
-                                      acc[mt][4 * g + 3] + bb.w);
-        }
+      for (int s = 0; s < 8; ++s) {
+        const int mt = s >> 1, g = 2 * (s & 1);
+        const float4 bl = *(const float4*)(bias + 32 * mt + 8 * g + 4 * h);
+        const float4 bh = *(const float4*)(bias + 32 * mt + 8 * (g + 1) + 4 * h);
+        b[s] = silu_bf16x8(make_float4(acc[mt][4 * g] + bl.x, acc[mt][4 * g + 1] + bl.y, acc[mt][4 * g + 2] + bl.z,
+                                       acc[mt][4 * g + 3] + bl.w),
+                           make_float4(acc[mt][4 * g + 4] + bh.x, acc[mt][4 * g + 5] + bh.y,
+                                       acc[mt][4 * g + 6] + bh.z, acc[mt][4 * g + 7] + bh.w));
+      }
     }
     float16_t y;
 #pragma unroll
     for (int x = 0; x < 16; ++x) y[x] = 0.f;
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const short4_t a = *(const short4_t*)(w4t + r * kLdsRow + 8 * s + 4 * h);
-      y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[s], y, 0, 0, 0);
+    for (int s = 0; s < 8; ++s) {
+      const bf16x8_t a = *(const bf16x8_t*)(w4t + r * kLdsRow + 16 * s + 8 * h);
+      y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], y, 0, 0, 0);
     }
     // lane (r, h) holds outputs 4 h + i (y[i]) and 8 + 4 h + i (y[4 + i]) of entry r, i = 0..3
     float o[kMlpOut];
@@ -614,7 +670,7 @@ hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const Atmo
   SG_TRY(hipGetLastError());
   const long long tiles = (entries + 31) / 32;
   const long long groups = (tiles + kFusedWaves - 1) / kFusedWaves;
-  const unsigned grid = (unsigned)std::min<long long>(groups, 1024);
+  const unsigned grid = (unsigned)std::min<long long>(groups, 512);  // 2 resident workgroups per CU
   PredTables pt;
   for (int k = 0; k < kMlpOrders; ++k) pt.p[k] = out[k];
   hipLaunchKernelGGL(fused_predict_kernel, dim3(grid), dim3(64 * kFusedWaves), 0, stream, P, s.theta_bf,
