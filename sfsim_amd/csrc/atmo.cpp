@@ -71,6 +71,13 @@ struct sfsim_atmo_ctx {
   sfsim_atmo_params graph_params;
   DevParams graph_dev;
   unsigned long long alloc_gen = 0, graph_gen = 0;
+  // graph of the last network-seeded build
+  hipGraphExec_t seeded_exec = nullptr;
+  sfsim_atmo_params seeded_params;
+  DevParams seeded_dev;
+  unsigned long long seeded_gen = 0;
+  int seeded_m = 0;
+  const void* seeded_bufs[3] = {nullptr, nullptr, nullptr};
   // timing
   std::vector<hipEvent_t> events;
   std::vector<int> event_kind;
@@ -555,10 +562,10 @@ AtmosphereCode code_of(const sfsim_atmo_params* p) {
 // `tol` the plain iteration runs instead.  The final ray-scatter table then follows the plain
 // iteration's accumulation, re-sampled order by order, on P_1..P_j of the seed and the computed
 // P_{j+1}..P_5.
-// host_sums (nullable): the order sums P_1..P_5 given by the caller (5 RGB scatter tables) instead of
-// the network's.
-int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double tol, double* residual, int* fallback,
-               const float* host_sums = nullptr) {
+
+// Validation and buffers of a seeded build (everything that allocates or copies from the host but the
+// caller's order sums).
+int prepare_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, const float* host_sums) {
   TRY(validate(ctx, p));
   if (p->iterations != kMlpOrders) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "the surrogate stands in for a 5-order set");
   if (m < 1 || m > kMlpOrders - 1) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "applications must be in 1..4");
@@ -590,8 +597,22 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
     TRY(ensure(ctx, ctx->out[i], outc[i] * sizeof(float)));
     ctx->out_count[i] = outc[i];
   }
+  TRY(ensure(ctx, ctx->geom, direction_geometry_bytes(P, ctx->sphere_n)));
+  const size_t slab = nS / (size_t)p->scatter_shape[1] * ctx->sphere_n * sizeof(float4);  // NH x dirs x NL x NA
+  TRY(ensure(ctx, ctx->slab, slab));
+  TRY(ensure(ctx, ctx->slab_mie, slab));
+  if (!host_sums) HIP_TRY(ctx, surrogate_reserve_tables(ctx->sg, P));
   ctx->built = false;
-  Timer timer{ctx, true};
+  return 0;
+}
+
+// The launches of a seeded build: T, E0, R1/M1, direction geometry, the seed (network prediction, or the
+// caller's sums uploaded), m applications, the residual and the final tables.  With host_sums == nullptr
+// there is no host copy, allocation or memset, so the sequence can be captured.
+int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevParams& P, int m, const float* host_sums,
+                   Timer& timer) {
+  const size_t nE = (size_t)count2(p->surface_radiance_shape);
+  const size_t nS = (size_t)count4(p->scatter_shape);
   auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
   const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
   const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
@@ -600,9 +621,8 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), nullptr, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), nullptr, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), nullptr, ctx->stream));
-  TRY(timer.begin(kDirectionGeometry));
-  TRY(prepare_point_scatter(ctx, P, true, nullptr));
-  TRY(timer.end());
+  LAUNCH(timer, kDirectionGeometry,
+         launch_direction_geometry(P, sdirs, ctx->sphere_n, ctx->geom.ptr, nullptr, ctx->stream));
   // the network's order sums: P_1..P_4 into pred[0..3] (the ones below the seed feed the final
   // table's re-sampling chain), its P_5 into dS[1] (residual)
   float4* want[kMlpOrders];
@@ -654,17 +674,6 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
   HIP_TRY(ctx, launch_residual(f4(ctx->R1), f4(ctx->pred[kMlpOrders - 1]), f4(ctx->dS[1]), n4, 1e-6f,
                                (float*)ctx->resid.ptr, ctx->stream));
   TRY(timer.end());
-  float rr[8];
-  HIP_TRY(ctx, hipMemcpyAsync(rr, ctx->resid.ptr, sizeof(rr), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  const float r = rr[4];
-  *residual = r;
-  ctx->last_residual = r;
-  if (!(r <= tol)) {
-    *fallback = 1;
-    return run_pipeline(ctx, p, nullptr, true);
-  }
-  *fallback = 0;
   // the plain iteration's accumulation S_k = tab(S_{k-1} + dS_k), S_0 = R1 (atmosphere_lut.clj:95-97), on
   // the orders dS_k = P_k - P_{k-1}: re-sampling at forward(backward(i)) is not the identity where the
   // sun direction is clamped, so the order-by-order chain is what the plain tables hold there
@@ -685,8 +694,70 @@ int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double to
          launch_resample(P, 3, f4(ctx->dE[1]), nullptr, nullptr, (float*)ctx->out[1].ptr, 0, ctx->stream));
   LAUNCH(timer, kFinal, launch_resample(P, 4, S_cur, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
   LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  TRY(timer.collect());
+  return 0;
+}
+
+void drop_seeded_graph(sfsim_atmo_ctx* ctx) {
+  if (ctx->seeded_exec) (void)hipGraphExecDestroy(ctx->seeded_exec);
+  ctx->seeded_exec = nullptr;
+}
+
+// host_sums (nullable): the order sums P_1..P_5 given by the caller (5 RGB scatter tables) instead of
+// the network's; that build runs kernel by kernel with event timings.  The network-seeded build is
+// captured into a graph on its first call with given parameters and m and replayed afterwards (the
+// weights are read at replay, so training in between needs no re-capture).
+int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double tol, double* residual, int* fallback,
+               const float* host_sums = nullptr) {
+  TRY(prepare_seeded(ctx, p, m, host_sums));
+  const DevParams P = make_dev_params(p);
+  if (host_sums) {
+    Timer timer{ctx, true};
+    TRY(enqueue_seeded(ctx, p, P, m, host_sums, timer));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    TRY(timer.collect());
+  } else {
+    const void* bufs[3] = {ctx->sg.theta, ctx->sg.theta_bf, ctx->sg.ztab};
+    const bool hit = ctx->seeded_exec && ctx->seeded_gen == ctx->alloc_gen && ctx->seeded_m == m &&
+                     std::memcmp(ctx->seeded_bufs, bufs, sizeof(bufs)) == 0 &&
+                     std::memcmp(&ctx->seeded_params, p, sizeof(*p)) == 0 &&
+                     std::memcmp(&ctx->seeded_dev, &P, sizeof(P)) == 0;
+    if (!hit) {
+      drop_seeded_graph(ctx);
+      HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      Timer timer{ctx, false};
+      const int rc = enqueue_seeded(ctx, p, P, m, nullptr, timer);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc != 0) {
+        if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIP_TRY(ctx, e);
+      const hipError_t ei = hipGraphInstantiate(&ctx->seeded_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        ctx->seeded_exec = nullptr;
+        return hip_fail(ctx, "hipGraphInstantiate", ei);
+      }
+      ctx->seeded_gen = ctx->alloc_gen;
+      ctx->seeded_m = m;
+      std::memcpy(ctx->seeded_bufs, bufs, sizeof(bufs));
+      ctx->seeded_params = *p;
+      ctx->seeded_dev = P;
+    }
+    HIP_TRY(ctx, hipGraphLaunch(ctx->seeded_exec, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  float rr[8];
+  HIP_TRY(ctx, hipMemcpy(rr, ctx->resid.ptr, sizeof(rr), hipMemcpyDeviceToHost));
+  const float r = rr[4];
+  *residual = r;
+  ctx->last_residual = r;
+  if (!(r <= tol)) {
+    *fallback = 1;
+    return run_pipeline_graph(ctx, p);
+  }
+  *fallback = 0;
   ctx->built = true;
   return 0;
 }
@@ -766,6 +837,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
+  drop_seeded_graph(ctx);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -654,9 +654,7 @@ hipError_t surrogate_make_targets(const float4* const* sums, long long entries, 
   return hipGetLastError();
 }
 
-hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* const* out,
-                                    hipStream_t stream) {
-  const long long entries = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
+hipError_t surrogate_reserve_tables(Surrogate& s, const DevParams& P) {
   const int rows = 1 + P.nS[0] + P.nS[1] + P.nS[2] + P.nS[3];
   if ((size_t)rows * kMlpHidden > s.ztab_capacity) {
     if (s.ztab) SG_TRY(hipFree(s.ztab));
@@ -665,6 +663,14 @@ hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const Atmo
     SG_TRY(dalloc(&s.ztab, (size_t)rows * kMlpHidden));
     s.ztab_capacity = (size_t)rows * kMlpHidden;
   }
+  return hipSuccess;
+}
+
+hipError_t surrogate_predict_tables(Surrogate& s, const DevParams& P, const AtmosphereCode& code, float4* const* out,
+                                    hipStream_t stream) {
+  const long long entries = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
+  const int rows = 1 + P.nS[0] + P.nS[1] + P.nS[2] + P.nS[3];
+  SG_TRY(surrogate_reserve_tables(s, P));
   hipLaunchKernelGGL(layer1_tables_kernel, dim3(rows), dim3(kMlpHidden), 0, stream, P, code, s.theta_bf, s.theta,
                      s.ztab);
   SG_TRY(hipGetLastError());

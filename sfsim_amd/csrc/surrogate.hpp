@@ -100,6 +100,8 @@ hipError_t surrogate_gather_batch(Surrogate& s, const DevParams& P, const Atmosp
 hipError_t surrogate_init_output_bias(Surrogate& s, const float* targets_dev, long long rows, hipStream_t stream);
 // Targets of one atmosphere from its order sums P_1..P_5 (float4 tables).
 hipError_t surrogate_make_targets(const float4* const* sums, long long entries, float* targets_dev, hipStream_t stream);
+// Allocates the layer-1 tables of a prediction over P's scatter grid (before a graph capture).
+hipError_t surrogate_reserve_tables(Surrogate& s, const DevParams& P);
 // Predicts the order sums P_k for every entry of the scatter grid of one atmosphere into out[k - 1]
 // (entries of `out` may be null: not written): layer-1 tables, then one fused kernel for layers 2-4
 // and the decoding.
