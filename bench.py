@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-seeded", action="store_true", help="skip the surrogate-seeded leg")
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=24000)
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: min(16, cores))")
     return ap.parse_args()
 
@@ -126,7 +126,7 @@ def seeded_leg(atm, ctx, P, workload, steps, train_steps):
     family = [P] + [atm.random_atmosphere(rng, P) for _ in range(7)]
     sg = atm.Surrogate(ctx, seed=1, batch=8192)
     t0 = time.perf_counter()
-    losses = sg.train(family, steps=train_steps, lr=2e-3, seed=2)
+    losses = sg.train(family, steps=train_steps, lr=3e-3, seed=2)
     train_s = time.perf_counter() - t0
     out = {"train": {"atmospheres": len(family), "steps": train_steps, "batch": 8192, "seconds": round(train_s, 2),
                      "loss_first": float(losses[0]), "loss_last": float(np.mean(losses[-20:]))}}

@@ -11,6 +11,9 @@
 // lowest point is interior and below DevParams::graze_height.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "kernels.hpp"
 
 namespace sfsim {
@@ -564,6 +567,7 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
       s_wt[threadIdx.x] = make_float2(Le.s, Lh.s);
     }
     __syncthreads();
+#pragma unroll 2
     for (int q = 0; q < qn; ++q) {
       const int4 o = s_off[q];
       const float2 w = s_wt[q];
@@ -578,8 +582,10 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       if (threadIdx.x + e * blockDim.x < NLA) {
-        // a heading coordinate with zero weight (every entry but the clamped ones) reads one column
-        if (La[e].s == 0.0f)
+        // a heading coordinate with zero weight (every entry but the clamped ones) reads one column;
+        // the choice is made per wavefront (the general lookup gives the same bits for such entries),
+        // so a wavefront never runs both loops
+        if (__all(La[e].s == 0.0f))
           acc[e] = ray_scatter_samples<true>(P, s_slab, s_w, s_g, s_rinv, i0, qn, NL, NA, NLA, vl[e], xl[e], La[e], acc[e]);
         else
           acc[e] = ray_scatter_samples<false>(P, s_slab, s_w, s_g, s_rinv, i0, qn, NL, NA, NLA, vl[e], xl[e], La[e], acc[e]);
@@ -592,7 +598,7 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
   unsigned long long lookups = 0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    int k = threadIdx.x + e * blockDim.x;
+    const int k = threadIdx.x + e * blockDim.x;
     if (k < NLA) {
       out[base + k] = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
       lookups += n;
@@ -1050,10 +1056,16 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
   return hipGetLastError();
 }
 
+// Samples per folded-slab chunk of the ray-scatter kernel: about 2048 slab entries (32 KB of LDS) by
+// default; SFSIM_ATMO_RAY_CHUNK (1..8) overrides it.
 int ray_scatter_chunk(const DevParams& P) {
   int nla = P.nS[2] * P.nS[3];
   int q = 2048 / nla;
-  return q < 1 ? 1 : (q > 8 ? 8 : q);
+  if (const char* env = std::getenv("SFSIM_ATMO_RAY_CHUNK")) {
+    int v = std::atoi(env);
+    if (v >= 1) q = v;
+  }
+  return q < 1 ? 1 : (q > kMaxRayChunk ? kMaxRayChunk : q);
 }
 
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
