@@ -191,31 +191,41 @@ bool finite_floats(const float* v, size_t n) {
   return true;
 }
 
-// Cubic through the 4 Chebyshev nodes of [0, xh], xh = (Rt^2 - R^2) / R^2, of
-// f(x) = k (R / 2) p(x), p(x) = 2 (sqrt(1 + x) - 1) / x = 2 / (1 + sqrt(1 + x)), so that
-// k h = x f(x) for the altitude h = sqrt(R^2 + x R^2) - R.  Relative error of the fit: 4e-10 for a
-// 100 km atmosphere on Earth, 4e-8 at the largest xh = 0.1 the parameters may have.
-void altitude_fit(double R, double Rt, double k, double out[4]) {
+// Polynomial of degree `deg` (2 or 3) through the deg + 1 Chebyshev nodes of [0, xh],
+// xh = (Rt^2 - R^2) / R^2, of f(x) = k (R / 2) p(x), p(x) = 2 (sqrt(1 + x) - 1) / x = 2 / (1 + sqrt(1 + x)),
+// so that k h = x f(x) for the altitude h = sqrt(R^2 + x R^2) - R.  Returns the largest relative error of
+// the fit over 2001 points of [0, xh].  Cubic: 4e-10 for a 100 km atmosphere on Earth, 4e-8 at the
+// largest xh = 0.1 the parameters may have; quadratic: 7e-8 for 100 km.
+double altitude_fit(double R, double Rt, double k, int deg, double out[4]) {
   const double xh = (Rt * Rt - R * R) / (R * R);
-  double A[4][5];
-  for (int i = 0; i < 4; ++i) {
-    const double x = 0.5 * xh * (1.0 + std::cos((2 * i + 1) * M_PI / 8.0));
+  const int m = deg + 1;
+  auto f = [&](double x) { return k * 0.5 * R * 2.0 / (1.0 + std::sqrt(1.0 + x)); };
+  double A[4][5] = {};
+  for (int i = 0; i < m; ++i) {
+    const double x = 0.5 * xh * (1.0 + std::cos((2 * i + 1) * M_PI / (2.0 * m)));
     double v = 1.0;
-    for (int j = 0; j < 4; ++j, v *= x) A[i][j] = v;
-    A[i][4] = k * 0.5 * R * 2.0 / (1.0 + std::sqrt(1.0 + x));
+    for (int j = 0; j < m; ++j, v *= x) A[i][j] = v;
+    A[i][4] = f(x);
   }
-  for (int c = 0; c < 4; ++c) {  // Gauss-Jordan with partial pivoting
+  for (int c = 0; c < m; ++c) {  // Gauss-Jordan with partial pivoting
     int piv = c;
-    for (int r = c + 1; r < 4; ++r)
+    for (int r = c + 1; r < m; ++r)
       if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
     for (int j = 0; j < 5; ++j) std::swap(A[c][j], A[piv][j]);
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < m; ++r) {
       if (r == c) continue;
-      const double f = A[r][c] / A[c][c];
-      for (int j = c; j < 5; ++j) A[r][j] -= f * A[c][j];
+      const double q = A[r][c] / A[c][c];
+      for (int j = c; j < 5; ++j) A[r][j] -= q * A[c][j];
     }
   }
-  for (int j = 0; j < 4; ++j) out[j] = A[j][4] / A[j][j];
+  for (int j = 0; j < 4; ++j) out[j] = j < m ? A[j][4] / A[j][j] : 0.0;
+  double worst = 0.0;
+  for (int i = 0; i <= 2000; ++i) {
+    const double x = xh * i / 2000.0;
+    const double y = out[0] + x * (out[1] + x * (out[2] + x * out[3]));
+    worst = std::fmax(worst, std::fabs(y / f(x) - 1.0));
+  }
+  return worst;
 }
 
 DevParams make_dev_params(const sfsim_atmo_params* p) {
@@ -240,8 +250,10 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   P.k_ray = (float)(-log2e / p->rayleigh.scatter_scale);
   P.inv_R = 1.0 / P.g.R;
   P.ray_over_mie = (float)(p->mie.scatter_scale / p->rayleigh.scatter_scale);
-  altitude_fit(P.g.R, P.g.Rt, -log2e / p->mie.scatter_scale, P.alt_poly_d);
-  for (int j = 0; j < 4; ++j) P.alt_poly[j] = (float)P.alt_poly_d[j];
+  altitude_fit(P.g.R, P.g.Rt, -log2e / p->mie.scatter_scale, 3, P.alt_poly_d);
+  double quad[4];
+  P.alt_deg = altitude_fit(P.g.R, P.g.Rt, -log2e / p->mie.scatter_scale, 2, quad) <= 1e-7 ? 2 : 3;
+  for (int j = 0; j < 4; ++j) P.alt_poly[j] = (float)(P.alt_deg == 2 ? quad[j] : P.alt_poly_d[j]);
   P.inv_scale_mie = 1.0 / p->mie.scatter_scale;
   P.inv_scale_ray = 1.0 / p->rayleigh.scatter_scale;
   P.g_mie = (float)p->mie.scatter_g;

@@ -20,7 +20,8 @@ struct DevParams {
   // h k_mie = x (a0 + x (a1 + x (a2 + x a3))), a cubic fit at Chebyshev nodes on [0, (Rt^2 - R^2) / R^2]
   // of (R / 2) k_mie 2 (sqrt(1 + x) - 1) / x; the Rayleigh exponent is ray_over_mie times it
   float alt_poly[4];
-  double alt_poly_d[4];  // the same cubic in FP64 (FP64 sums)
+  int alt_deg;           // 2 when the quadratic fit is within 1e-7 relative on the whole range, else 3
+  double alt_poly_d[4];  // the cubic in FP64 (FP64 sums)
   float ray_over_mie;
   double inv_R;
   double inv_scale_mie, inv_scale_ray;

@@ -27,33 +27,21 @@ __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// FP64 square root and reciprocal from the FP32 hardware estimates and Newton steps in FP64 (relative
-// error ~1e-14, against ~1e-16 for the correctly rounded library sequences that cost several times as
-// many instructions).  Arguments outside the FP32 normal range take the library routines.
-__device__ __forceinline__ double sqrt_newton(double a) {
-  if (!(a > 1e-30 && a < 1e30)) return sqrt(a);
-  double y = (double)__builtin_amdgcn_rsqf((float)a);  // 1 / sqrt(a) to ~1e-7
-  y = y * fma(-0.5 * a * y, y, 1.5);                    // ~1e-14
-  return a * y;
-}
-
-__device__ __forceinline__ double rcp_newton(double a) {
-  if (!(a > 1e-30 && a < 1e30)) return 1.0 / a;
-  double z = (double)__builtin_amdgcn_rcpf((float)a);  // ~1e-7
-  z = z * fma(-a, z, 2.0);                              // ~1e-14
-  return z;
-}
-
-// top_exit_distance (geometry.hpp) with sqrt_newton / rcp_newton.
-__device__ __forceinline__ double top_exit_distance_dev(const Geo& g, double h, double pd) {
-  double c = (g.H - h) * (2.0 * g.R + g.H + h);
-  double disc = pd * pd + c;
-  if (disc > 0) {
-    double l2 = sqrt_newton(disc);
-    double d = pd > 0 ? c * rcp_newton(pd + l2) : l2 - pd;
-    return fmax(0.0, d);
-  }
-  return fmax(0.0, -pd);
+// top_exit_distance (geometry.hpp) for a point inside the atmosphere (h <= H), branch-free, with the
+// FP64 square root and reciprocal from the FP32 hardware estimates refined by one Newton step each
+// (relative error ~1e-14, against ~1e-16 for the correctly rounded library sequences that cost several
+// times as many instructions).  The discriminant pd^2 + (Rt^2 - r^2) lies in [0, 1e14]; it is clamped
+// below at 1e-30 (an exit distance under 1e-14 m in place of the zero it stands for) so that the FP32
+// estimates stay in their normal range.
+__device__ __forceinline__ double sun_exit_distance(const Geo& g, double h, double pd) {
+  const double c = (g.H - h) * (2.0 * g.R + g.H + h);
+  const double disc = fmax(fma(pd, pd, c), 1e-30);
+  double y = (double)__builtin_amdgcn_rsqf((float)disc);
+  y = y * fma(-0.5 * disc * y, y, 1.5);
+  const double l2 = disc * y;
+  double z = (double)__builtin_amdgcn_rcpf((float)(pd + l2));
+  z = z * fma(-(pd + l2), z, 2.0);
+  return pd > 0.0 ? c * z : l2 - pd;
 }
 
 __device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
@@ -99,11 +87,17 @@ __device__ __forceinline__ Segment make_segment(const DevParams& P, double h0, d
 // Sums over the n midpoint nodes of exp(-h / H_mie) and exp(-h / H_rayleigh) with FP32 altitudes;
 // 8-node partial sums, Kahan-compensated across them.  Node j sits at s_j = s0 + j dt (units of R,
 // measured from the segment's lowest point), x_j = c0 + s_j (b + s_j) = (r_j^2 - R^2) / R^2, and the
-// Mie exponent log2(exp(-h_j / H_mie)) is x_j times a cubic in x_j (DevParams::alt_poly), so a node
-// costs five FMAs, two multiplies, an add and the two exponentials.
-__device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) {
+// Mie exponent log2(exp(-h_j / H_mie)) is x_j times a polynomial of degree DEG in x_j
+// (DevParams::alt_poly, alt_deg), so a node costs DEG + 3 FMAs, two multiplies, an add and the two
+// exponentials.
+template <int DEG>
+__device__ float2 density_sums_f32_deg(const DevParams& P, const Segment& g, int n) {
   const float a0 = P.alt_poly[0], a1 = P.alt_poly[1], a2 = P.alt_poly[2], a3 = P.alt_poly[3];
   const float rho = P.ray_over_mie;
+  auto exponent = [&](float x) {
+    const float p = DEG == 3 ? fmaf(fmaf(fmaf(a3, x, a2), x, a1), x, a0) : fmaf(fmaf(a2, x, a1), x, a0);
+    return x * p;
+  };
   float sm = 0.f, sr = 0.f, cm = 0.f, cr = 0.f;
   int j = 0;
   for (; j + 8 <= n; j += 8) {
@@ -112,8 +106,7 @@ __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) 
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const float s = fmaf((float)q, g.dt, base);
-      const float x = fmaf(s, s + g.b, g.c0);
-      const float em = x * fmaf(fmaf(fmaf(a3, x, a2), x, a1), x, a0);
+      const float em = exponent(fmaf(s, s + g.b, g.c0));
       const float m = fexp2(em), r = fexp2(em * rho);
       if (q == 0) {
         pm = m;
@@ -130,8 +123,7 @@ __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) 
     float pm = 0.f, pr = 0.f;
     for (; j < n; ++j) {
       const float s = fmaf((float)j, g.dt, g.s0);
-      const float x = fmaf(s, s + g.b, g.c0);
-      const float em = x * fmaf(fmaf(fmaf(a3, x, a2), x, a1), x, a0);
+      const float em = exponent(fmaf(s, s + g.b, g.c0));
       pm += fexp2(em);
       pr += fexp2(em * rho);
     }
@@ -139,6 +131,10 @@ __device__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) 
     kahan_add(sr, cr, pr);
   }
   return make_float2(sm, sr);
+}
+
+__device__ __forceinline__ float2 density_sums_f32(const DevParams& P, const Segment& g, int n) {
+  return P.alt_deg == 2 ? density_sums_f32_deg<2>(P, g, n) : density_sums_f32_deg<3>(P, g, n);
 }
 
 // Same sums with FP64 node positions, altitude exponents and accumulation (near-horizon and grazing
@@ -430,7 +426,7 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
       const int owner = item >> 8, i = i0 + (int)(item & 255);
       const double h = s_h[i];
       const double pl = s_xl[owner] + s_t[i] * s_vl[owner];  // dot(p_i, l)
-      const double Ds = top_exit_distance_dev(P.g, h, pl);
+      const double Ds = sun_exit_distance(P.g, h, pl);
       const float3 Ts = segment_transmittance_f32(P, make_segment(P, h, pl, Ds, n), Ds, n);
       s_res[it] = Ts.x;
       s_res[it + kSSWave * kSSChunk] = Ts.y;
@@ -441,7 +437,7 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
       const int owner = item >> 8, i = i0 + (int)(item & 255);
       const double h = s_h[i];
       const double pl = s_xl[owner] + s_t[i] * s_vl[owner];
-      const double Ds = top_exit_distance_dev(P.g, h, pl);
+      const double Ds = sun_exit_distance(P.g, h, pl);
       const float3 Ts = segment_transmittance_f64(P, make_segment(P, h, pl, Ds, n), Ds, n);
       s_res[it] = Ts.x;
       s_res[it + kSSWave * kSSChunk] = Ts.y;
