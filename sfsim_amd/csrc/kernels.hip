@@ -862,10 +862,20 @@ __global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int
     Lerp Le = make_lerp((float)elevation_to_index(P.g, NE, r0, r0 * se, above), NE);
     Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
     Lerp La = make_lerp((float)sun_angle_to_index(NA, se * l0 + ce * l1), NA);
-    v = lookup4(a, NE, NL, NA, Lh, Le, Ls, La);
-    if (b) {
-      float3 w = lookup4(b, NE, NL, NA, Lh, Le, Ls, La);
-      v.x += w.x; v.y += w.y; v.z += w.z;
+    if (Lh.s == 0.0f && Le.s == 0.0f && Ls.s == 0.0f && La.s == 0.0f) {
+      // the entry maps onto a grid point (every weight zero): the lookup is that point's value, bit for bit
+      const long long c = (((long long)Lh.i0 * NE + Le.i0) * NL + Ls.i0) * NA + La.i0;
+      v = ld3(a, c);
+      if (b) {
+        const float3 w = ld3(b, c);
+        v.x += w.x; v.y += w.y; v.z += w.z;
+      }
+    } else {
+      v = lookup4(a, NE, NL, NA, Lh, Le, Ls, La);
+      if (b) {
+        float3 w = lookup4(b, NE, NL, NA, Lh, Le, Ls, La);
+        v.x += w.x; v.y += w.y; v.z += w.z;
+      }
     }
     // image.clj:299-312: row h * NL + s, column e * NA + a
     dst = tiled ? ((long long)(ih * NL + is) * (NE * NA) + (long long)ie * NA + ia) : idx;
@@ -886,9 +896,9 @@ __global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int
     }
     Lerp L0 = make_lerp((float)height_to_index(P.g, N0, r0), N0);
     Lerp L1 = make_lerp((float)c1, N1);
-    v = lookup2(a, N1, L0, L1);
+    v = lookup2_skip(a, N1, L0, L1);
     if (b) {
-      float3 w = lookup2(b, N1, L0, L1);
+      float3 w = lookup2_skip(b, N1, L0, L1);
       v.x += w.x; v.y += w.y; v.z += w.z;
     }
     dst = idx;
