@@ -1,0 +1,169 @@
+"""The oracle's first-order tables against an independent brute-force evaluator written here with numpy:
+straight-line geometry, midpoint sums along the view ray and toward the sun, no tables and no shared
+code beyond the index spaces (which the reference's own known answers pin).  Tiny grids, FP64 on both
+sides: the only differences are summation order and the rounding of the sphere intersections.  Also the sphere-quadrature identities and
+forward(backward(i)) = i on feasible grid points."""
+import numpy as np
+import pytest
+
+from tests.parity import oracle_params
+
+
+@pytest.fixture(scope="module")
+def atm():
+    from sfsim_amd import atmosphere  # parameter structs only (no device call)
+    return atmosphere
+
+R, H = 6378000.0, 100000.0
+MIE = dict(base=np.full(3, 2e-5), scale=1200.0, quotient=0.9, g=0.76)
+RAY = dict(base=np.array([5.8e-6, 13.5e-6, 33.1e-6]), scale=8000.0, quotient=1.0, g=0.0)
+
+
+def _params(atm, shape, steps):
+    return atm.make_params(planet={"height": H}, transmittance_shape=(5, 7), surface_radiance_shape=(4, 6),
+                           scatter_shape=shape, ray_steps=steps, sphere_steps=8, iterations=0)
+
+
+def altitude(p):
+    return np.linalg.norm(p) - R
+
+
+def extinction(h):
+    return (RAY["base"] / RAY["quotient"]) * np.exp(-h / RAY["scale"]) + \
+        (MIE["base"] / MIE["quotient"]) * np.exp(-h / MIE["scale"])
+
+
+def exit_distance(p, d, radius):
+    """Far intersection of the ray p + t d with the sphere |q| = radius (p inside)."""
+    b = p @ d
+    return -b + np.sqrt(b * b - (p @ p - radius * radius))
+
+
+def ground_distance(p, d):
+    b = p @ d
+    disc = b * b - (p @ p - R * R)
+    return -b - np.sqrt(disc) if (b < 0 and disc > 0) else None
+
+
+def transmittance(p, q, n):
+    seg = q - p
+    length = np.linalg.norm(seg)
+    if length == 0.0:
+        return np.ones(3)
+    t = (np.arange(n) + 0.5) / n
+    tau = sum(extinction(altitude(p + ti * seg)) for ti in t) * (length / n)
+    return np.exp(-tau)
+
+
+def sun_visible(p, l):
+    b = p @ l
+    return b >= 0 or (p @ p - b * b) >= R * R
+
+
+def phase(g, mu):
+    return 3 * (1 - g * g) * (1 + mu * mu) / (8 * np.pi * (2 + g * g) * (1 + g * g - 2 * g * mu) ** 1.5)
+
+
+def single_scatter(x, v, l, above, n):
+    """(R1, M1): Rayleigh in-scattering with its phase, Mie strength without phase, intensity 1."""
+    D = exit_distance(x, v, R + H) if above else ground_distance(x, v)
+    if D is None:
+        D = max(0.0, -(x @ v))
+    r1, m1 = np.zeros(3), np.zeros(3)
+    for i in range(n):
+        p = x + (i + 0.5) * D / n * v
+        if not sun_visible(p, l):
+            continue
+        h = altitude(p)
+        sun = p + exit_distance(p, l, R + H) * l
+        w = transmittance(x, p, n) * transmittance(p, sun, n) * D / n
+        r1 += w * RAY["base"] * np.exp(-h / RAY["scale"]) * phase(0.0, v @ l)
+        m1 += w * MIE["base"] * np.exp(-h / MIE["scale"])
+    return r1, m1
+
+
+@pytest.mark.parametrize("steps", [4, 7])
+def test_single_scatter_against_brute_force(atm, oracle, steps):
+    shape = (3, 5, 4, 3)
+    P = _params(atm, shape, steps)
+    OP = oracle_params(oracle, P)
+    idx = np.arange(int(np.prod(shape)))
+    r1, m1 = oracle.stage_single_scatter(OP, 0, idx), oracle.stage_single_scatter(OP, 1, idx)
+    planet = oracle.planet(R, H)
+    worst = 0.0
+    for k, ijkl in enumerate(np.ndindex(*shape)):
+        x, v, l, above = oracle.space_backward(planet, oracle.SPACE_SCATTER, shape, [float(i) for i in ijkl])
+        br, bm = single_scatter(np.asarray(x), np.asarray(v), np.asarray(l), above, steps)
+        for got, ref in ((r1[k], br), (m1[k], bm)):
+            scale = max(np.max(np.abs(ref)), 1e-30)
+            worst = max(worst, float(np.max(np.abs(got - ref)) / scale))
+    # FP64 on both sides; the sphere intersections round |p|^2 - r^2 differently near the boundaries
+    # (points at the top of the atmosphere), which reaches ~1e-7 relative: a wrong phase, a missing
+    # transmittance factor or a wrong endpoint would be O(1)
+    assert worst < 1e-6, worst
+
+
+def test_transmittance_against_brute_force(atm, oracle):
+    P = _params(atm, (3, 5, 4, 3), 6)
+    OP = oracle_params(oracle, P)
+    shape = tuple(P.transmittance_shape)
+    T = oracle.stage_transmittance(OP)
+    planet = oracle.planet(R, H)
+    for k, ij in enumerate(np.ndindex(*shape)):
+        x, v, _, above = oracle.space_backward(planet, oracle.SPACE_TRANSMITTANCE, shape, [float(i) for i in ij])
+        x, v = np.asarray(x), np.asarray(v)
+        D = exit_distance(x, v, R + H) if above else ground_distance(x, v)
+        if D is None:
+            D = max(0.0, -(x @ v))
+        ref = transmittance(x, x + D * v, 6)
+        # same rounding of the sphere intersections as in the single-scatter comparison
+        assert np.allclose(T[k], ref, rtol=1e-6, atol=0.0), (ij, T[k], ref)
+
+
+@pytest.mark.parametrize("steps", [8, 14, 15, 28, 64])
+def test_sphere_quadrature_identities(oracle, steps):
+    """integral-sphere of a constant is 4 pi exactly (the ring weights telescope).  integral-half-sphere
+    of the direction itself points along the normal and tends to pi n; the latitude-ring midpoint rule
+    overshoots by about 5/steps^2 (measured: 8.2e-2 at 8 rings, 6.3e-3 at 28, 1.2e-3 at 64), so the
+    bound 8/steps^2 fails for a dropped cosine weight or a ring offset by half a step."""
+    for normal in ((1.0, 0.0, 0.0), (0.0, 0.0, 1.0), tuple(np.array([1.0, 2.0, 2.0]) / 3.0)):
+        n = np.asarray(normal)
+        full = oracle.integral_sphere_test(False, steps, normal, 0)
+        assert np.allclose(full, 4 * np.pi, rtol=1e-12), (steps, normal, full)
+        half = oracle.integral_sphere_test(True, steps, normal, 1)
+        along = float(half @ n)
+        assert np.linalg.norm(half - along * n) < 1e-12 * along, (steps, normal, half)
+        assert 0.0 < along / np.pi - 1.0 < 8.0 / steps ** 2, (steps, normal, along / np.pi)
+
+
+def test_forward_of_backward_is_identity_on_grid_points(oracle):
+    """Index spaces: forward(backward(i)) = i on the well-conditioned grid points.  Two families are
+    singular in the reference's spaces and left out: height index 0 (on the ground every direction
+    below the horizon has zero length and maps to the horizon) and the horizon index itself (the
+    above/below split).  The scatter sun index is not an identity where limit-quot clamps the sun
+    direction; at this shape 40% of the points keep it exactly."""
+    planet = oracle.planet(R, H)
+    shape = (5, 8)
+    for ij in np.ndindex(*shape):
+        x, v, l, above = oracle.space_backward(planet, oracle.SPACE_SURFACE_RADIANCE, shape, [float(i) for i in ij])
+        f = oracle.space_forward(planet, oracle.SPACE_SURFACE_RADIANCE, shape, x, v, l, above)
+        assert np.allclose(f, ij, atol=1e-9), (ij, f)
+    for shape in ((6, 9), (7, 17)):
+        for ij in np.ndindex(*shape):
+            if ij[0] == 0 or 2 * ij[1] == shape[1] - 1:
+                continue
+            x, v, l, above = oracle.space_backward(planet, oracle.SPACE_TRANSMITTANCE, shape, [float(i) for i in ij])
+            f = oracle.space_forward(planet, oracle.SPACE_TRANSMITTANCE, shape, x, v, l, above)
+            # near the horizon the elevation goes through a square root of a small difference: ~1e-7
+            assert np.allclose(f, ij, atol=1e-6), (shape, ij, f)
+    shape = (4, 6, 5, 4)
+    exact = total = 0
+    for ijkl in np.ndindex(*shape):
+        if ijkl[0] == 0 or 2 * ijkl[1] == shape[1] - 1:
+            continue
+        x, v, l, above = oracle.space_backward(planet, oracle.SPACE_SCATTER, shape, [float(i) for i in ijkl])
+        f = oracle.space_forward(planet, oracle.SPACE_SCATTER, shape, x, v, l, above)
+        assert np.allclose(f[:3], ijkl[:3], atol=1e-9), (ijkl, f)
+        exact += bool(abs(f[3] - ijkl[3]) < 1e-6)
+        total += 1
+    assert exact == 144 and total == 360, (exact, total)
