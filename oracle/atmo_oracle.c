@@ -21,10 +21,16 @@
  *     reference's table-level probes at size 12 (t_atmosphere.clj:629-775,
  *     t_planet.clj:312-376), evaluated through the GLSL-consumer restatement
  *     at the end of this file.
- *   - point-scatter (J) and orders >= 2 of the driver: parity unpinned by
- *     reference-held vectors; pinned by invariants only (closed-form ground
- *     and constant-radiance cases, scaling in the scattering coefficient,
- *     the optically thin limit).  See tests/test_oracle_invariants.py.
+ *   - point-scatter (J): no reference-held vector; pinned by a brute-force
+ *     physics evaluator that computes the first-order radiance and the ground
+ *     irradiance directly (tests/test_oracle_bruteforce.py: the difference is
+ *     interpolation error and shrinks with the grid) and by invariants
+ *     (closed-form ground and constant-radiance cases, scaling in the
+ *     scattering coefficient, the optically thin limit;
+ *     tests/test_oracle_invariants.py).
+ *   - orders >= 2 of the driver: parity unpinned by reference-held vectors;
+ *     composed of the pinned stages, with the bookkeeping rebuilt from the
+ *     stage functions in tests/test_oracle_invariants.py.
  */
 #include <math.h>
 #include <stdint.h>

@@ -167,3 +167,92 @@ def test_forward_of_backward_is_identity_on_grid_points(oracle):
         exact += bool(abs(f[3] - ijkl[3]) < 1e-6)
         total += 1
     assert exact == 144 and total == 360, (exact, total)
+
+
+# ---------------------------------------------------------------- point scatter (second order)
+BRIGHTNESS = 0.3  # planet brightness of the default parameters (albedo)
+
+
+def sphere_nodes(steps, normal):
+    """Latitude-ring midpoint nodes of integral-sphere (sphere.clj:62-105) in numpy, in the frame whose
+    first row is the normal and whose second is normal x (the unit axis least aligned with it)."""
+    n = np.asarray(normal, np.float64)
+    o1 = np.cross(n, np.eye(3)[int(np.argmin(np.abs(n)))])
+    o1 /= np.linalg.norm(o1)
+    o2 = np.cross(n, o1)
+    rings = steps >> 1
+    dirs, weights = [], []
+    for i in range(rings):
+        theta = np.pi * (0.5 + i) / rings
+        band = np.cos(theta - np.pi / rings / 2) - np.cos(theta + np.pi / rings / 2)
+        count = int(np.ceil(np.sin(theta) * steps))
+        phi = 2 * np.pi * (0.5 + np.arange(count)) / count
+        dirs.append(np.cos(theta) * n + np.sin(theta) * (np.outer(np.cos(phi), o1) + np.outer(np.sin(phi), o2)))
+        weights.append(np.full(count, band * 2 * np.pi / count))
+    return np.concatenate(dirs), np.concatenate(weights), np.stack([n, o1, o2])
+
+
+def surface_radiance_base(p, l, n):
+    normal = p / np.linalg.norm(p)
+    return transmittance(p, p + exit_distance(p, l, R + H) * l, n) * max(0.0, normal @ l)
+
+
+def point_scatter_first(x, v, l, n, sphere_steps):
+    """J[S1, E0](x, v, l) from physics: over the sphere around x, the in-scattering toward v of the
+    first-order radiance arriving from each direction (evaluated directly, no table) plus, for
+    directions that hit the ground, the ground's reflection of the direct sun light."""
+    dirs, weights, _ = sphere_nodes(sphere_steps, x / np.linalg.norm(x))
+    h = altitude(x)
+    beta_r = RAY["base"] * np.exp(-h / RAY["scale"])
+    beta_m = MIE["base"] * np.exp(-h / MIE["scale"])
+    total = np.zeros(3)
+    for omega, w in zip(dirs, weights):
+        ground = ground_distance(x, omega)
+        r1, m1 = single_scatter(x, omega, l, ground is None, n)
+        radiance = r1 + m1 * phase(MIE["g"], omega @ l)
+        if ground is not None:
+            p = x + ground * omega
+            radiance = radiance + transmittance(x, p, n) * (BRIGHTNESS / np.pi) * surface_radiance_base(p, l, n)
+        total += w * (beta_r * phase(0.0, v @ omega) + beta_m * phase(MIE["g"], v @ omega)) * radiance
+    return total
+
+
+@pytest.mark.parametrize("steps", [8, 15])
+def test_sphere_nodes_are_the_oracle_quadrature(oracle, steps):
+    for normal in ((0.0, 0.0, 1.0), tuple(np.array([1.0, 2.0, 2.0]) / 3.0), (0.6, 0.0, 0.8)):
+        dirs, weights, frame = sphere_nodes(steps, normal)
+        assert np.allclose(frame, oracle.oriented_matrix(normal), atol=1e-15)
+        assert len(dirs) == oracle.sphere_direction_count(False, steps)
+        for kind, f in ((1, dirs), (2, np.column_stack([np.ones(len(dirs)), dirs[:, 1:]]))):
+            ref = oracle.integral_sphere_test(False, steps, normal, kind)
+            assert np.allclose(weights @ f, ref, rtol=0, atol=1e-12), (steps, normal, kind)
+
+
+def test_point_scatter_converges_to_brute_force(atm, oracle):
+    """The oracle's point scatter reads the first-order radiance from interpolated 4D tables and the
+    ground's direct irradiance from the interpolated 2D table; the evaluator above computes both
+    directly.  The difference is interpolation error only, so it must shrink as the grids refine
+    (median over daylight grid points: 5.1e-2 at (9, 17, 9, 5), 2.2e-2 at (17, 33, 17, 9)).
+    A dropped ground term (median 1.1), the view-sun angle in place of the view-ray angle in the
+    phase functions (0.27) or albedo without the 1/pi (0.36) stay O(0.3) at every grid."""
+    planet = oracle.planet(R, H)
+    medians = []
+    for shape in ((9, 17, 9, 5), (17, 33, 17, 9)):
+        P = atm.make_params(planet={"height": H}, transmittance_shape=(9, 17),
+                            surface_radiance_shape=(shape[0], shape[2]), scatter_shape=shape, ray_steps=8,
+                            sphere_steps=8, iterations=0)
+        OP = oracle_params(oracle, P)
+        r1, m1 = oracle.stage_single_scatter(OP, 0), oracle.stage_single_scatter(OP, 1)
+        e0 = oracle.stage_surface_radiance_base(OP)
+        rng = np.random.default_rng(3)
+        errors = []
+        while len(errors) < 10:
+            ijkl = (int(rng.integers(1, shape[0])), int(rng.integers(0, shape[1])),
+                    int(rng.integers(shape[2] // 3, shape[2])), int(rng.integers(0, shape[3])))
+            J = oracle.stage_point_scatter(OP, 1, r1, m1, e0, indices=[int(np.ravel_multi_index(ijkl, shape))])[0]
+            x, v, l, _ = oracle.space_backward(planet, oracle.SPACE_SCATTER, shape, [float(i) for i in ijkl])
+            ref = point_scatter_first(np.asarray(x), np.asarray(v), np.asarray(l), 8, 8)
+            errors.append(float(np.max(np.abs(J - ref) / np.abs(ref))))
+        medians.append(float(np.median(errors)))
+        print(shape, "median rel %.3e, max %.3e" % (medians[-1], max(errors)))
+    assert medians[1] < 0.04 and medians[1] < 0.6 * medians[0], medians
