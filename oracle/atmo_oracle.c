@@ -28,9 +28,11 @@
  *     (closed-form ground and constant-radiance cases, scaling in the
  *     scattering coefficient, the optically thin limit;
  *     tests/test_oracle_invariants.py).
- *   - orders >= 2 of the driver: parity unpinned by reference-held vectors;
- *     composed of the pinned stages, with the bookkeeping rebuilt from the
- *     stage functions in tests/test_oracle_invariants.py.
+ *   - order 2 of the driver: the same brute-force evaluator carried one order
+ *     further (tests/test_oracle_bruteforce.py).  Orders >= 3: parity
+ *     unpinned by reference-held vectors; they repeat the pinned stages, with
+ *     the bookkeeping rebuilt from the stage functions in
+ *     tests/test_oracle_invariants.py.
  */
 #include <math.h>
 #include <stdint.h>
