@@ -30,7 +30,7 @@ EXPORTS = (
     "sfsim_atmo_sphere_directions", "sfsim_atmo_surrogate_size", "sfsim_atmo_surrogate_reset",
     "sfsim_atmo_surrogate_weights", "sfsim_atmo_surrogate_set_weights", "sfsim_atmo_surrogate_eval",
     "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_build_from_sums", "sfsim_atmo_order_sums",
-    "sfsim_atmo_surrogate_predict",
+    "sfsim_atmo_surrogate_predict", "sfsim_atmo_set_sun_table", "sfsim_atmo_get_sun_table",
 )
 
 
@@ -164,6 +164,8 @@ def lib():
                                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         L.sfsim_atmo_order_sums.argtypes = [vp, pp, fp]
         L.sfsim_atmo_surrogate_predict.argtypes = [vp, pp, fp]
+        L.sfsim_atmo_set_sun_table.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.sfsim_atmo_get_sun_table.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -231,6 +233,19 @@ class Atmosphere:
     def _check(self, rc):
         if rc:
             raise AtmoError(rc, lib().sfsim_atmo_last_error(self._ctx).decode())
+
+    @property
+    def sun_table(self):
+        """(rows, cols) of the single-scatter stage's sun optical-depth table; (0, 0): every sun
+        segment summed node by node (``sfsim_atmo_set_sun_table``)."""
+        r, c = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().sfsim_atmo_get_sun_table(self._ctx, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
+    @sun_table.setter
+    def sun_table(self, shape):
+        rows, cols = shape
+        self._check(lib().sfsim_atmo_set_sun_table(self._ctx, int(rows), int(cols)))
 
     # ---------------------------------------------------------------- tables
     def transmittance_table(self, P):

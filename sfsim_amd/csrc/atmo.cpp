@@ -45,6 +45,10 @@ struct DeviceBuffer {
 
 }  // namespace
 
+// Default size of the sun optical-depth table: 1024 horizon-distance rows x 1024 exit-distance columns
+// (8 MiB); bilinear interpolation of its log2 sums reproduces the per-segment sums to |d tau| <= 1e-4.
+constexpr int kSunRows = 1024, kSunCols = 1024;
+
 struct sfsim_atmo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -61,6 +65,9 @@ struct sfsim_atmo_ctx {
   DeviceBuffer sphere_dirs, half_dirs;
   DeviceBuffer geom, slab, slab_mie;  // point-scatter direction geometry and folded dS slabs
   int sphere_key = -1, half_key = -1, sphere_n = 0, half_n = 0;
+  // single scatter's sun optical-depth table (rows x cols of log2 node sums; 0 x 0 sums every segment)
+  int sun_rows = kSunRows, sun_cols = kSunCols;
+  DeviceBuffer sun_depth;
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
   Surrogate sg;
@@ -278,6 +285,22 @@ DevParams make_dev_params(const sfsim_atmo_params* p) {
   return P;
 }
 
+// make_dev_params plus the context's sun optical-depth table (allocated by ensure_sun_depth).
+DevParams dev_params(const sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  DevParams P = make_dev_params(p);
+  if (ctx->sun_rows > 0) {
+    P.sun_rows = ctx->sun_rows;
+    P.sun_cols = ctx->sun_cols;
+    P.sun_depth = (float2*)ctx->sun_depth.ptr;
+  }
+  return P;
+}
+
+int ensure_sun_depth(sfsim_atmo_ctx* ctx) {
+  if (ctx->sun_rows <= 0) return 0;
+  return ensure(ctx, ctx->sun_depth, (size_t)ctx->sun_rows * ctx->sun_cols * sizeof(float2));
+}
+
 // sphere.clj:62-105 for the normal (1 0 0) of every grid point: rings theta_i = range (i + 1/2) / N,
 // weight (cos(theta - d) - cos(theta + d)) 2 pi / K_i, K_i = ceil(sin(theta_i) steps), phi_j = 2 pi (j + 1/2) / K_i,
 // direction x n + y o1 + z o2 with the frame of oriented-matrix (matrix.clj:207-213, quaternion.clj:166-171).
@@ -400,6 +423,7 @@ struct Timer {
 // from the host, so that the launches that follow can be captured into a graph.
 int prepare_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(validate(ctx, p));
+  TRY(ensure_sun_depth(ctx));
   if (p->iterations > 0) {
     TRY(validate_sphere(ctx, p->sphere_steps));
     TRY(validate_half_sphere(ctx, p->ray_steps));
@@ -508,7 +532,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
 int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* wc, bool timed,
                  float4* const* capture = nullptr) {
   TRY(prepare_pipeline(ctx, p));
-  const DevParams P = make_dev_params(p);
+  const DevParams P = dev_params(ctx, p);
   Timer timer{ctx, timed};
   TRY(enqueue_pipeline(ctx, p, P, wc, timer, capture));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -526,7 +550,7 @@ void drop_graph(sfsim_atmo_ctx* ctx) {
 // buffers, replayed afterwards (the kernels are deterministic, so replays equal eager builds).
 int run_pipeline_graph(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(prepare_pipeline(ctx, p));
-  const DevParams P = make_dev_params(p);
+  const DevParams P = dev_params(ctx, p);
   const bool hit = ctx->graph_exec && ctx->graph_gen == ctx->alloc_gen &&
                    std::memcmp(&ctx->graph_params, p, sizeof(*p)) == 0 &&
                    std::memcmp(&ctx->graph_dev, &P, sizeof(P)) == 0;
@@ -588,7 +612,8 @@ int prepare_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, const
   TRY(validate_half_sphere(ctx, p->ray_steps));
   TRY(ensure_dirs(ctx, 0, p->sphere_steps));
   TRY(ensure_dirs(ctx, 1, p->ray_steps));
-  const DevParams P = make_dev_params(p);
+  TRY(ensure_sun_depth(ctx));
+  const DevParams P = dev_params(ctx, p);
   const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
   const size_t nS = (size_t)count4(p->scatter_shape);
   TRY(ensure(ctx, ctx->T, nT * sizeof(float4)));
@@ -721,7 +746,7 @@ void drop_seeded_graph(sfsim_atmo_ctx* ctx) {
 int run_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, double tol, double* residual, int* fallback,
                const float* host_sums = nullptr) {
   TRY(prepare_seeded(ctx, p, m, host_sums));
-  const DevParams P = make_dev_params(p);
+  const DevParams P = dev_params(ctx, p);
   if (host_sums) {
     Timer timer{ctx, true};
     TRY(enqueue_seeded(ctx, p, P, m, host_sums, timer));
@@ -845,7 +870,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
@@ -861,11 +886,28 @@ const char* sfsim_atmo_last_error(const sfsim_atmo_ctx* ctx) {
   return ctx->error.c_str();
 }
 
+int sfsim_atmo_set_sun_table(sfsim_atmo_ctx* ctx, int rows, int cols) {
+  if (!ctx) return SFSIM_ATMO_ERR_ARGUMENT;
+  const bool per_segment = rows == 0 && cols == 0;
+  if (!per_segment && (rows < 16 || cols < 16 || rows > 8192 || cols > 8192))
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sun table rows and cols must both be 0 or both in [16, 8192]");
+  ctx->sun_rows = rows;
+  ctx->sun_cols = cols;
+  return 0;
+}
+
+int sfsim_atmo_get_sun_table(const sfsim_atmo_ctx* ctx, int* rows, int* cols) {
+  if (!ctx || !rows || !cols) return SFSIM_ATMO_ERR_ARGUMENT;
+  *rows = ctx->sun_rows;
+  *cols = ctx->sun_cols;
+  return 0;
+}
+
 int sfsim_atmo_transmittance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* out) {
   TRY(activate(ctx));
   TRY(validate(ctx, p));
   if (!out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "out is NULL");
-  DevParams P = make_dev_params(p);
+  DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count2(p->transmittance_shape);
   int rc;
   float4* o = stage_out(ctx, n, &rc);
@@ -878,7 +920,7 @@ int sfsim_atmo_surface_radiance_base_table(sfsim_atmo_ctx* ctx, const sfsim_atmo
   TRY(activate(ctx));
   TRY(validate(ctx, p));
   if (!out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "out is NULL");
-  DevParams P = make_dev_params(p);
+  DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count2(p->surface_radiance_shape);
   int rc;
   float4* o = stage_out(ctx, n, &rc);
@@ -891,7 +933,8 @@ int sfsim_atmo_single_scatter_tables(sfsim_atmo_ctx* ctx, const sfsim_atmo_param
                                      float* mie_out) {
   TRY(activate(ctx));
   TRY(validate(ctx, p));
-  DevParams P = make_dev_params(p);
+  TRY(ensure_sun_depth(ctx));
+  DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count4(p->scatter_shape);
   TRY(ensure(ctx, ctx->R1, n * sizeof(float4)));
   TRY(ensure(ctx, ctx->M1, n * sizeof(float4)));
@@ -908,7 +951,7 @@ int sfsim_atmo_point_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params*
   TRY(validate_sphere(ctx, p->sphere_steps));
   if (!dS || !dE || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dS, dE and out must not be NULL");
   TRY(ensure_dirs(ctx, 0, p->sphere_steps));
-  DevParams P = make_dev_params(p);
+  DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count4(p->scatter_shape), ne = (size_t)count2(p->surface_radiance_shape);
   const float4 *ddS, *dM, *ddE;
   TRY(upload_table(ctx, 0, dS, n, &ddS));
@@ -927,7 +970,7 @@ int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p
   TRY(activate(ctx));
   TRY(validate(ctx, p));
   if (!dJ || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dJ and out must not be NULL");
-  DevParams P = make_dev_params(p);
+  DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count4(p->scatter_shape);
   const float4* ddJ;
   TRY(upload_table(ctx, 0, dJ, n, &ddJ));
@@ -945,7 +988,7 @@ int sfsim_atmo_surface_radiance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_para
   TRY(validate_half_sphere(ctx, p->ray_steps));
   if (!dS || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dS and out must not be NULL");
   TRY(ensure_dirs(ctx, 1, p->ray_steps));
-  DevParams P = make_dev_params(p);
+  DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count4(p->scatter_shape), ne = (size_t)count2(p->surface_radiance_shape);
   const float4 *ddS, *dM;
   TRY(upload_table(ctx, 0, dS, n, &ddS));
@@ -964,7 +1007,7 @@ int sfsim_atmo_resample_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, i
   TRY(validate(ctx, p));
   if (space != 2 && space != 3 && space != 4) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "space must be 2, 3 or 4");
   if (!a || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "a and out must not be NULL");
-  DevParams P = make_dev_params(p);
+  DevParams P = dev_params(ctx, p);
   size_t n = space == 4   ? (size_t)count4(p->scatter_shape)
              : space == 2 ? (size_t)count2(p->transmittance_shape)
                           : (size_t)count2(p->surface_radiance_shape);
@@ -1135,7 +1178,7 @@ int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atm
   }
   HIP_TRY(ctx, hipMemcpyAsync(ctx->codes.ptr, codes.data(), count * sizeof(AtmosphereCode), hipMemcpyHostToDevice,
                               ctx->stream));
-  const DevParams P = make_dev_params(&p0);
+  const DevParams P = dev_params(ctx, &p0);
   Surrogate& sg = ctx->sg;
   // a fresh network (no Adam step taken yet) starts from the targets' means
   if (sg.adam_step == 0)
@@ -1218,7 +1261,7 @@ int sfsim_atmo_surrogate_predict(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p
     TRY(ensure(ctx, ctx->pred[k], nS * sizeof(float4)));
     out[k] = (float4*)ctx->pred[k].ptr;
   }
-  const DevParams P = make_dev_params(p);
+  const DevParams P = dev_params(ctx, p);
   HIP_TRY(ctx, surrogate_predict_tables(ctx->sg, P, code_of(p), out, ctx->stream));
   for (int k = 0; k < kMlpOrders; ++k) TRY(download_table(ctx, out[k], sums + (size_t)k * nS * 3, nS));
   return 0;

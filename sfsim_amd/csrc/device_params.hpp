@@ -38,6 +38,10 @@ struct DevParams {
   // graze_height (u = r^2 - R^2 < graze_u there) is accumulated in FP64 (near-horizon / grazing
   // rays); everything else in FP32 with compensated sums
   double graze_height, graze_u;
+  // single scatter: the sun segments' optical depths from a sun_rows x sun_cols table of log2 of their
+  // midpoint sums (sun_depth, built once per build); sun_rows == 0 sums every sun segment instead
+  int sun_rows, sun_cols;
+  float2* sun_depth;
 };
 
 // Quadrature direction (FP64 unit vector in the frame of a grid point on +x) and its weight.

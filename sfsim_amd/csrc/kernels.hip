@@ -476,6 +476,127 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
   count(wc, samples, samples + view, 0, 0);
 }
 
+// ------------------------------------------------------------ tabulated sun optical depths
+// The sun segment of a sample p (atmosphere.clj:154-160: transmittance from p toward l to the top of
+// the atmosphere, ray_steps midpoint nodes) depends on p and l only through r = |p| and pl = dot(p, l),
+// so its two node sums (Mie and Rayleigh density times the node spacing) are functions of two
+// variables.  Rows: a = rho / rho_max, rho = sqrt(r^2 - R^2), uniform in the horizon distance (the
+// altitude grid is finest at the ground); columns: b = (D - D_min) / (D_max - D_min) for the exit
+// distance D, D_min = Rt - r (sun at the zenith) and D_max = rho + rho_max (sun on the horizon), which
+// spans every direction in which the sun is above the horizon.  Entries hold log2 of the node sums
+// (FP64 for grazing segments, compensated FP32 otherwise); interpolation is bilinear in log2
+// (exponential in altitude between grid points).
+__global__ void __launch_bounds__(kBlock) sun_depth_kernel(const DevParams P) {
+  const int rows = P.sun_rows, cols = P.sun_cols;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const int ia = idx / cols, ib = idx - ia * cols;
+  const double rho = P.g.rho_max * ia / (rows - 1);
+  const double h = rho * rho / (P.g.R + sqrt(P.g.R * P.g.R + rho * rho));
+  const double c = (P.g.rho_max - rho) * (P.g.rho_max + rho);  // Rt^2 - r^2
+  const double dmin = P.g.H - h, dmax = rho + P.g.rho_max;
+  const double D = dmin + (dmax - dmin) * ib / (cols - 1);
+  float2 out = make_float2(-256.f, -256.f);  // empty segment (the top of the atmosphere, sun at the zenith)
+  if (D > 0.0) {
+    const double pl = (c - D * D) / (2.0 * D);  // from D = -pl + sqrt(pl^2 + c)
+    const Segment g = make_segment(P, h, pl, D, P.ray_steps);
+    double sm, sr;
+    if (__any(g.graze)) {  // the same FP32/FP64 choice as segment_transmittance
+      const double2 sum = density_sums_f64(P, g, P.ray_steps);
+      sm = sum.x * g.dtd;
+      sr = sum.y * g.dtd;
+    } else {
+      const float2 sum = density_sums_f32(P, g, P.ray_steps);
+      sm = (double)sum.x * g.dtd;
+      sr = (double)sum.y * g.dtd;
+    }
+    out = make_float2((float)fmax(log2(sm), -256.0), (float)fmax(log2(sr), -256.0));
+  }
+  P.sun_depth[idx] = out;
+}
+
+// Single scatter with the sun segments' optical depths read from the sun_depth table: one block per
+// view ray, one thread per (sun elevation, heading) entry, each adding its lit samples in sample order.
+// The view segments T(x, p_i) and the sample geometry are computed once per ray into LDS.
+constexpr int kSSTBlock = 256;
+
+__global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(const DevParams P, float4* __restrict__ R1,
+                                                                         float4* __restrict__ M1, WorkCounters* wc) {
+  extern __shared__ unsigned char s_raw[];
+  const int NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const int NLA = NL * NA;
+  const int n = P.ray_steps, cols = P.sun_cols;
+  double* s_t = (double*)s_raw;        // [n] sample distance along the ray
+  double* s_h = s_t + n;               // [n] sample altitude
+  float4* s_w = (float4*)(s_h + n);    // [n] T(x, p_i) rgb, w: exp(-h_i / H_mie)
+  float4* s_g = s_w + n;               // [n] Rt^2 - r_i^2, D_min, (cols - 1) / (D_max - D_min), exp(-h_i / H_rayleigh)
+  float2* s_r = (float2*)(s_g + n);    // [n] row fraction, first table index of the row (bits of an int)
+
+  const int rayid = blockIdx.x;
+  const int ih = rayid / NE, ie = rayid - ih * NE;
+  const Ray ray = make_ray(P, ih, ie);
+  const double dt = ray.D / n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double t = (i + 0.5) * dt;
+    const double h = altitude_along(P.g, ray.h0, ray.pd0, t);
+    s_t[i] = t;
+    s_h[i] = h;
+    const float3 T = segment_transmittance(P, ray.h0, ray.pd0, t, n);
+    s_w[i] = make_float4(T.x, T.y, T.z, (float)exp(-h * P.inv_scale_mie));
+    const double rho = sqrt(h * (2.0 * P.g.R + h));
+    const double dmin = P.g.H - h, dmax = rho + P.g.rho_max;
+    s_g[i] = make_float4((float)((P.g.H - h) * (2.0 * P.g.R + P.g.H + h)), (float)dmin,
+                         (float)((cols - 1) / (dmax - dmin)), (float)exp(-h * P.inv_scale_ray));
+    const float x = (float)(rho / P.g.rho_max * (P.sun_rows - 1));
+    const float row = fminf(floorf(x), (float)(P.sun_rows - 2));
+    s_r[i] = make_float2(x - row, __int_as_float((int)row * cols));
+  }
+  __syncthreads();
+  const float2* __restrict__ tab = P.sun_depth;
+  unsigned long long lit = 0;
+  for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
+    const int is = k / NA, ia = k - is * NA;
+    const double sin_s = index_to_sin_sun_elevation(NL, is);
+    double l0, l1, l2;
+    index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
+    const double vl = ray.se * l0 + ray.ce * l1;
+    const double xl = ray.r0 * l0;
+    float3 accR = make_float3(0.f, 0.f, 0.f), accM = make_float3(0.f, 0.f, 0.f);
+    for (int i = 0; i < n; ++i) {
+      const double h = s_h[i], pl = xl + s_t[i] * vl;  // dot(p_i, l)
+      if (!is_above_horizon(P.g, h, pl)) continue;  // atmosphere.clj:154-160
+      ++lit;
+      const float4 gm = s_g[i];
+      const float2 rr = s_r[i];
+      const float plf = (float)pl;
+      const float q = __builtin_amdgcn_sqrtf(fmaf(plf, plf, gm.x));
+      const float D = plf > 0.f ? gm.x * __builtin_amdgcn_rcpf(plf + q) : q - plf;
+      const float y = fminf(fmaxf((D - gm.y) * gm.z, 0.f), (float)(cols - 1));
+      const float col = fminf(floorf(y), (float)(cols - 2));
+      const float fy = y - col;
+      const int base = __float_as_int(rr.y) + (int)col;
+      const float2 v00 = tab[base], v01 = tab[base + 1], v10 = tab[base + cols], v11 = tab[base + cols + 1];
+      const float lo_m = fmaf(fy, v01.x - v00.x, v00.x), hi_m = fmaf(fy, v11.x - v10.x, v10.x);
+      const float lo_r = fmaf(fy, v01.y - v00.y, v00.y), hi_r = fmaf(fy, v11.y - v10.y, v10.y);
+      const float sm = fexp2(fmaf(rr.x, hi_m - lo_m, lo_m)), sr = fexp2(fmaf(rr.x, hi_r - lo_r, lo_r));
+      const float3 Ts = transmittance_of(P, sm, sr, 1.0);
+      const float4 w = s_w[i];
+      const float3 t3 = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
+      accR.x += t3.x * gm.w; accR.y += t3.y * gm.w; accR.z += t3.z * gm.w;
+      accM.x += t3.x * w.w; accM.y += t3.y * w.w; accM.z += t3.z * w.w;
+    }
+    const long long o = (long long)rayid * NLA + k;
+    const float a = (float)dt;
+    const float pr = phase_f(P.g_ray, (float)vl) * a;
+    R1[o] = make_float4(accR.x * P.sca_ray[0] * P.intensity[0] * pr, accR.y * P.sca_ray[1] * P.intensity[1] * pr,
+                        accR.z * P.sca_ray[2] * P.intensity[2] * pr, 0.f);
+    M1[o] = make_float4(accM.x * P.sca_mie[0] * P.intensity[0] * a, accM.y * P.sca_mie[1] * P.intensity[1] * a,
+                        accM.z * P.sca_mie[2] * P.intensity[2] * a, 0.f);
+  }
+  const unsigned long long view = (threadIdx.x < n) ? (unsigned long long)n * ((n - threadIdx.x + blockDim.x - 1) / blockDim.x) : 0;
+  count(wc, lit, view, 0, 0);
+}
+
 // dS' = ray-scatter of an interpolated dJ table (atmosphere_lut.clj:92,94; atmosphere.clj:192-200).
 // One workgroup per view ray; lanes over (sun elevation, heading) entries, up to EPT per lane.
 // At sample p_i the height and elevation coordinates of the dJ lookup are the same for every
@@ -1029,6 +1150,15 @@ hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCou
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
                                  hipStream_t stream) {
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
+  if (P.sun_rows > 0) {
+    if (!P.sun_depth || P.sun_rows < 2 || P.sun_cols < 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(sun_depth_kernel, dim3(blocks_for((long long)P.sun_rows * P.sun_cols)), dim3(kBlock), 0, stream, P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t shmem = (size_t)P.ray_steps * (2 * sizeof(double) + 2 * sizeof(float4) + sizeof(float2));
+    hipLaunchKernelGGL(single_scatter_table_kernel, dim3(rays), dim3(kSSTBlock), shmem, stream, P, rayleigh, mie, wc);
+    return hipGetLastError();
+  }
   unsigned groups = (unsigned)((P.nS[2] * P.nS[3] + kSSWave - 1) / kSSWave);
   int n = P.ray_steps;
   size_t shmem = (size_t)n * (2 * sizeof(double) + sizeof(float4)) + sizeof(float) * (size_t)((n + 3) & ~3) +
