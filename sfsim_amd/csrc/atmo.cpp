@@ -30,13 +30,14 @@ enum Kind {
   kAccumulate,
   kFinal,
   kSurrogate,
+  kRayGeometry,
   kNumKinds
 };
 
 const char* const kKindNames[kNumKinds] = {"transmittance",      "surface_radiance_base", "single_scatter",
                                            "direction_geometry", "point_scatter",         "surface_radiance",
                                            "ray_scatter",        "accumulate",            "final_resample",
-                                           "surrogate"};
+                                           "surrogate",          "ray_geometry"};
 
 struct DeviceBuffer {
   void* ptr = nullptr;
@@ -68,6 +69,7 @@ struct sfsim_atmo_ctx {
   // single scatter's sun optical-depth table (rows x cols of log2 node sums; 0 x 0 sums every segment)
   int sun_rows = kSunRows, sun_cols = kSunCols;
   DeviceBuffer sun_depth;
+  DeviceBuffer ray_geo;  // per-(view ray, sample) geometry of single and ray scatter
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
   Surrogate sg;
@@ -293,12 +295,19 @@ DevParams dev_params(const sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
     P.sun_cols = ctx->sun_cols;
     P.sun_depth = (float2*)ctx->sun_depth.ptr;
   }
+  P.ray_geo = (char*)ctx->ray_geo.ptr;
   return P;
 }
 
 int ensure_sun_depth(sfsim_atmo_ctx* ctx) {
   if (ctx->sun_rows <= 0) return 0;
   return ensure(ctx, ctx->sun_depth, (size_t)ctx->sun_rows * ctx->sun_cols * sizeof(float2));
+}
+
+// The sun table and the per-(view ray, sample) geometry buffer of the single/ray-scatter kernels.
+int ensure_scatter_buffers(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  TRY(ensure_sun_depth(ctx));
+  return ensure(ctx, ctx->ray_geo, ray_geometry_bytes(make_dev_params(p)));
 }
 
 // sphere.clj:62-105 for the normal (1 0 0) of every grid point: rings theta_i = range (i + 1/2) / N,
@@ -423,7 +432,7 @@ struct Timer {
 // from the host, so that the launches that follow can be captured into a graph.
 int prepare_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(validate(ctx, p));
-  TRY(ensure_sun_depth(ctx));
+  TRY(ensure_scatter_buffers(ctx, p));
   if (p->iterations > 0) {
     TRY(validate_sphere(ctx, p->sphere_steps));
     TRY(validate_half_sphere(ctx, p->ray_steps));
@@ -472,6 +481,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
 
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), wc, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), wc, ctx->stream));
+  LAUNCH(timer, kRayGeometry, launch_ray_geometry(P, wc, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream));
   if (p->iterations > 0)
     LAUNCH(timer, kDirectionGeometry,
@@ -612,7 +622,7 @@ int prepare_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, const
   TRY(validate_half_sphere(ctx, p->ray_steps));
   TRY(ensure_dirs(ctx, 0, p->sphere_steps));
   TRY(ensure_dirs(ctx, 1, p->ray_steps));
-  TRY(ensure_sun_depth(ctx));
+  TRY(ensure_scatter_buffers(ctx, p));
   const DevParams P = dev_params(ctx, p);
   const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
   const size_t nS = (size_t)count4(p->scatter_shape);
@@ -657,6 +667,7 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   const int j = kMlpOrders - m;  // seeded partial sum X_j, j >= 1
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), nullptr, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), nullptr, ctx->stream));
+  LAUNCH(timer, kRayGeometry, launch_ray_geometry(P, nullptr, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), nullptr, ctx->stream));
   LAUNCH(timer, kDirectionGeometry,
          launch_direction_geometry(P, sdirs, ctx->sphere_n, ctx->geom.ptr, nullptr, ctx->stream));
@@ -870,7 +881,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
@@ -933,12 +944,13 @@ int sfsim_atmo_single_scatter_tables(sfsim_atmo_ctx* ctx, const sfsim_atmo_param
                                      float* mie_out) {
   TRY(activate(ctx));
   TRY(validate(ctx, p));
-  TRY(ensure_sun_depth(ctx));
+  TRY(ensure_scatter_buffers(ctx, p));
   DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count4(p->scatter_shape);
   TRY(ensure(ctx, ctx->R1, n * sizeof(float4)));
   TRY(ensure(ctx, ctx->M1, n * sizeof(float4)));
   ctx->built = false;
+  HIP_TRY(ctx, launch_ray_geometry(P, nullptr, ctx->stream));
   HIP_TRY(ctx, launch_single_scatter(P, (float4*)ctx->R1.ptr, (float4*)ctx->M1.ptr, nullptr, ctx->stream));
   TRY(download_table(ctx, (const float4*)ctx->R1.ptr, rayleigh_out, n));
   return download_table(ctx, (const float4*)ctx->M1.ptr, mie_out, n);
@@ -970,6 +982,7 @@ int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p
   TRY(activate(ctx));
   TRY(validate(ctx, p));
   if (!dJ || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dJ and out must not be NULL");
+  TRY(ensure_scatter_buffers(ctx, p));
   DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count4(p->scatter_shape);
   const float4* ddJ;
@@ -977,6 +990,7 @@ int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p
   int rc;
   float4* o = stage_out(ctx, n, &rc);
   TRY(rc);
+  HIP_TRY(ctx, launch_ray_geometry(P, nullptr, ctx->stream));
   HIP_TRY(ctx, launch_ray_scatter(P, ddJ, o, nullptr, ctx->stream));
   return download_table(ctx, o, out, n);
 }

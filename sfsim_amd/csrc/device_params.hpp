@@ -42,7 +42,44 @@ struct DevParams {
   // midpoint sums (sun_depth, built once per build); sun_rows == 0 sums every sun segment instead
   int sun_rows, sun_cols;
   float2* sun_depth;
+  // per-(view ray, sample) geometry of the single-scatter and ray-scatter kernels, computed once per
+  // build (ray_geometry_bytes(P) bytes, laid out by ray_geo_view)
+  char* ray_geo;
 };
+
+// Views of DevParams::ray_geo: m = NH x NE x ray_steps samples, sample i of view ray (ih, ie) at
+// (ih NE + ie) ray_steps + i; t_i = (i + 1/2) D / ray_steps along the ray from the grid point.
+struct RayGeoView {
+  double2* th;  // t_i, altitude h_i
+  float4* T;    // T(x, p_i) rgb, w: exp(-h_i / H_mie)
+  float4* rs;   // height coordinate, elevation coordinate of v, alpha_i, beta_i (sun-elevation coordinate terms)
+  float4* ss;   // Rt^2 - r_i^2, D_min, (sun_cols - 1) / (D_max - D_min), exp(-h_i / H_rayleigh) (sun table)
+  float2* row;  // sun-table row fraction, bits of the int index of the row start
+};
+
+__host__ __device__ inline size_t ray_geo_samples(const DevParams& P) {
+  return (size_t)P.nS[0] * P.nS[1] * P.ray_steps;
+}
+
+__host__ __device__ inline size_t ray_geometry_bytes(const DevParams& P) {
+  return ray_geo_samples(P) * (sizeof(double2) + 3 * sizeof(float4) + sizeof(float2));
+}
+
+__host__ __device__ inline RayGeoView ray_geo_view(const DevParams& P) {
+  const size_t m = ray_geo_samples(P);
+  char* b = P.ray_geo;
+  RayGeoView v;
+  v.th = (double2*)b;
+  b += m * sizeof(double2);
+  v.T = (float4*)b;
+  b += m * sizeof(float4);
+  v.rs = (float4*)b;
+  b += m * sizeof(float4);
+  v.ss = (float4*)b;
+  b += m * sizeof(float4);
+  v.row = (float2*)b;
+  return v;
+}
 
 // Quadrature direction (FP64 unit vector in the frame of a grid point on +x) and its weight.
 struct Dir {

@@ -515,41 +515,68 @@ __global__ void __launch_bounds__(kBlock) sun_depth_kernel(const DevParams P) {
   P.sun_depth[idx] = out;
 }
 
+// Geometry of every (view ray, sample) pair, one thread each: the sample position, T(x, p_i) over
+// ray_steps midpoint nodes, the height and elevation coordinates of the dJ lookup, the terms of the
+// sun-elevation coordinate and, with a sun table, the sample's row and exit-distance terms in it.
+// Shared by the single-scatter kernel and the ray-scatter launches of all iterations.
+__global__ void __launch_bounds__(kBlock) ray_geometry_kernel(const DevParams P, WorkCounters* wc) {
+  const int n = P.ray_steps, NH = P.nS[0], NE = P.nS[1];
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)NH * NE * n) return;
+  const int rayid = (int)(g / n), i = (int)(g - (long long)rayid * n);
+  const int ih = rayid / NE, ie = rayid - ih * NE;
+  const Ray ray = make_ray(P, ih, ie);
+  const double t = (i + 0.5) * (ray.D / n);
+  const double h = altitude_along(P.g, ray.h0, ray.pd0, t), r = P.g.R + h;
+  const float3 T = segment_transmittance(P, ray.h0, ray.pd0, t, n);
+  const RayGeoView V = ray_geo_view(P);
+  const double k3 = -3.0 * 1.4426950408889634;
+  V.th[g] = make_double2(t, h);
+  V.T[g] = make_float4(T.x, T.y, T.z, (float)exp(-h * P.inv_scale_mie));
+  V.rs[g] = make_float4((float)altitude_to_index(P.g, NH, h),
+                        (float)elevation_to_index(P.g, NE, r, ray.pd0 + t, ray.above), (float)(k3 / r),
+                        (float)(k3 * t / r));
+  if (P.sun_rows > 0) {
+    const int cols = P.sun_cols;
+    const double rho = sqrt(h * (2.0 * P.g.R + h));
+    const double dmin = P.g.H - h, dmax = rho + P.g.rho_max;
+    V.ss[g] = make_float4((float)((P.g.H - h) * (2.0 * P.g.R + P.g.H + h)), (float)dmin,
+                          (float)((cols - 1) / (dmax - dmin)), (float)exp(-h * P.inv_scale_ray));
+    const float x = (float)(rho / P.g.rho_max * (P.sun_rows - 1));
+    const float row = fminf(floorf(x), (float)(P.sun_rows - 2));
+    V.row[g] = make_float2(x - row, __int_as_float((int)row * cols));
+  }
+  count(wc, 0, n, 0, 0);
+}
+
 // Single scatter with the sun segments' optical depths read from the sun_depth table: one block per
 // view ray, one thread per (sun elevation, heading) entry, each adding its lit samples in sample order.
 // The view segments T(x, p_i) and the sample geometry are computed once per ray into LDS.
 constexpr int kSSTBlock = 256;
 
-__global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(const DevParams P, float4* __restrict__ R1,
-                                                                         float4* __restrict__ M1, WorkCounters* wc) {
-  extern __shared__ unsigned char s_raw[];
+__global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
+    const DevParams P, const double2* __restrict__ geo_th, const float4* __restrict__ geo_T,
+    const float4* __restrict__ geo_ss, const float2* __restrict__ geo_row, float4* __restrict__ R1,
+    float4* __restrict__ M1, WorkCounters* wc) {
   const int NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps, cols = P.sun_cols;
-  double* s_t = (double*)s_raw;        // [n] sample distance along the ray
-  double* s_h = s_t + n;               // [n] sample altitude
-  float4* s_w = (float4*)(s_h + n);    // [n] T(x, p_i) rgb, w: exp(-h_i / H_mie)
-  float4* s_g = s_w + n;               // [n] Rt^2 - r_i^2, D_min, (cols - 1) / (D_max - D_min), exp(-h_i / H_rayleigh)
-  float2* s_r = (float2*)(s_g + n);    // [n] row fraction, first table index of the row (bits of an int)
-
   const int rayid = blockIdx.x;
   const int ih = rayid / NE, ie = rayid - ih * NE;
   const Ray ray = make_ray(P, ih, ie);
   const double dt = ray.D / n;
+  // this ray's samples from the per-build geometry, staged in LDS
+  extern __shared__ unsigned char s_raw[];
+  double2* s_th = (double2*)s_raw;      // [n] t_i, h_i
+  float4* s_w = (float4*)(s_th + n);    // [n] T(x, p_i) rgb, w: exp(-h_i / H_mie)
+  float4* s_g = s_w + n;                // [n] Rt^2 - r_i^2, D_min, (cols - 1) / (D_max - D_min), exp(-h_i / H_rayleigh)
+  float2* s_r = (float2*)(s_g + n);     // [n] sun-table row fraction, start of the row (bits of an int)
+  const long long g0 = (long long)rayid * n;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double t = (i + 0.5) * dt;
-    const double h = altitude_along(P.g, ray.h0, ray.pd0, t);
-    s_t[i] = t;
-    s_h[i] = h;
-    const float3 T = segment_transmittance(P, ray.h0, ray.pd0, t, n);
-    s_w[i] = make_float4(T.x, T.y, T.z, (float)exp(-h * P.inv_scale_mie));
-    const double rho = sqrt(h * (2.0 * P.g.R + h));
-    const double dmin = P.g.H - h, dmax = rho + P.g.rho_max;
-    s_g[i] = make_float4((float)((P.g.H - h) * (2.0 * P.g.R + P.g.H + h)), (float)dmin,
-                         (float)((cols - 1) / (dmax - dmin)), (float)exp(-h * P.inv_scale_ray));
-    const float x = (float)(rho / P.g.rho_max * (P.sun_rows - 1));
-    const float row = fminf(floorf(x), (float)(P.sun_rows - 2));
-    s_r[i] = make_float2(x - row, __int_as_float((int)row * cols));
+    s_th[i] = geo_th[g0 + i];
+    s_w[i] = geo_T[g0 + i];
+    s_g[i] = geo_ss[g0 + i];
+    s_r[i] = geo_row[g0 + i];
   }
   __syncthreads();
   const float2* __restrict__ tab = P.sun_depth;
@@ -563,7 +590,8 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(const D
     const double xl = ray.r0 * l0;
     float3 accR = make_float3(0.f, 0.f, 0.f), accM = make_float3(0.f, 0.f, 0.f);
     for (int i = 0; i < n; ++i) {
-      const double h = s_h[i], pl = xl + s_t[i] * vl;  // dot(p_i, l)
+      const double2 th = s_th[i];
+      const double h = th.y, pl = xl + th.x * vl;  // dot(p_i, l)
       if (!is_above_horizon(P.g, h, pl)) continue;  // atmosphere.clj:154-160
       ++lit;
       const float4 gm = s_g[i];
@@ -593,8 +621,7 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(const D
     M1[o] = make_float4(accM.x * P.sca_mie[0] * P.intensity[0] * a, accM.y * P.sca_mie[1] * P.intensity[1] * a,
                         accM.z * P.sca_mie[2] * P.intensity[2] * a, 0.f);
   }
-  const unsigned long long view = (threadIdx.x < n) ? (unsigned long long)n * ((n - threadIdx.x + blockDim.x - 1) / blockDim.x) : 0;
-  count(wc, lit, view, 0, 0);
+  count(wc, lit, 0, 0, 0);
 }
 
 // dS' = ray-scatter of an interpolated dJ table (atmosphere_lut.clj:92,94; atmosphere.clj:192-200).
@@ -603,53 +630,82 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(const D
 // entry of the ray, so the workgroup first folds those two dimensions into a (sun elevation,
 // heading) slab in LDS (coalesced reads of four dJ slabs), Q samples at a time; each lane then
 // interpolates its own sun-elevation / heading coordinates in that slab.
-// One entry's samples i0 .. i0 + qn - 1 of the folded slabs: sun-elevation coordinate from
-// dot(p_i, l) / |p_i|, heading lerp La fixed along the ray.
+// One entry's samples i0 .. i0 + qn - 1 of the folded slabs.  The sun-elevation coordinate of
+// sun-elevation-to-index (atmosphere.clj:322-326) at p_i, (NL - 1) max(0, (1 - exp(-3 s - 0.6)) / (1 - exp(-3.6)))
+// with s = dot(p_i, l) / |p_i| = (xl + t_i vl) / r_i, is A - A exp2(alpha_i xl + beta_i vl + c) with
+// A = (NL - 1) / (1 - exp(-3.6)), alpha_i = -3 log2(e) / r_i, beta_i = alpha_i t_i, c = -0.6 log2(e), clamped
+// to [0, NL - 1] (interpolate.clj:75-98 clips the coordinate); the lower row is min(floor, NL - 2), so
+// the upper row is always the next one (weight 1 on it at the clamped top).  LDS byte offsets are
+// formed in FP32 (exact below 2^24).
+struct SunCoord {
+  float A, c, top;  // A, c, NL - 1
+};
+
+__device__ __forceinline__ float sun_coord(const SunCoord& sc, float2 ab, float xl, float vl) {
+  const float e = fexp2(fmaf(ab.y, vl, fmaf(ab.x, xl, sc.c)));
+  return __builtin_amdgcn_fmed3f(fmaf(-sc.A, e, sc.A), 0.0f, sc.top);
+}
+
+__device__ __forceinline__ float3 lerp3(float4 a, float4 b, float s) {
+  return make_float3(fmaf(s, b.x - a.x, a.x), fmaf(s, b.y - a.y, a.y), fmaf(s, b.z - a.z, a.z));
+}
+
+__device__ __forceinline__ float4 lds4(const float4* base, unsigned byte_offset) {
+  return *(const float4*)((const char*)base + byte_offset);
+}
+
 template <bool kColumn>
-__device__ __forceinline__ float3 ray_scatter_samples(const DevParams& P, const float4* s_slab, const float4* s_w,
-                                                      const float2* s_g, const float* s_rinv, int i0, int qn, int NL,
-                                                      int NA, int NLA, float vl, float xl, Lerp La, float3 acc) {
-  for (int q = 0; q < qn; ++q) {
+__device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const float4* s_slab,
+                                                      const float4* __restrict__ gT, const float4* __restrict__ gRS,
+                                                      int i0, int qn, float row_bytes, float col_bytes, int NLA,
+                                                      float vl, float xl, Lerp La, float3 acc) {
+  const unsigned rb = (unsigned)row_bytes;
+  const float slab_bytes = (float)(NLA * (int)sizeof(float4));
+  float q_bytes = col_bytes;  // byte offset of the entry's column in slab q
+  for (int q = 0; q < qn; ++q, q_bytes += slab_bytes) {
     const int i = i0 + q;
-    const float4 w = s_w[i];
-    const float sin_e = fmaf(s_g[i].y, vl, xl) * s_rinv[i];
-    const Lerp Ls = make_lerp(sun_elevation_coord(P, NL, sin_e), NL);
+    const float4 w = gT[i];  // workgroup-uniform: scalar loads
+    const float4 g = gRS[i];
+    const float c = sun_coord(sc, make_float2(g.z, g.w), xl, vl);
+    const float row = fminf(floorf(c), sc.top - 1.0f);
+    const float s = c - row;
+    const unsigned off = (unsigned)fmaf(row, row_bytes, q_bytes);
     float3 v;
     if (kColumn) {
-      const float4* col = s_slab + q * NLA + La.i0;
-      v = mix3(ld3(col, Ls.i0 * NA), ld3(col, Ls.i1 * NA), Ls.s);
+      v = lerp3(lds4(s_slab, off), lds4(s_slab, off + rb), s);
     } else {
-      v = lookup2(s_slab + q * NLA, NA, Ls, La);
+      const unsigned d = (unsigned)(La.i1 - La.i0) * (unsigned)sizeof(float4);
+      const float3 u = lerp3(lds4(s_slab, off), lds4(s_slab, off + d), La.s);
+      const float3 t = lerp3(lds4(s_slab, off + rb), lds4(s_slab, off + rb + d), La.s);
+      v = make_float3(fmaf(s, t.x - u.x, u.x), fmaf(s, t.y - u.y, u.y), fmaf(s, t.z - u.z, u.z));
     }
-    acc.x += w.x * v.x; acc.y += w.y * v.y; acc.z += w.z * v.z;
+    acc.x = fmaf(w.x, v.x, acc.x);
+    acc.y = fmaf(w.y, v.y, acc.y);
+    acc.z = fmaf(w.z, v.z, acc.z);
   }
   return acc;
 }
 
 template <int EPT>
 __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
+                                                             const float4* __restrict__ geo_T,
+                                                             const float4* __restrict__ geo_rs,
                                                              float4* __restrict__ out, int Q, WorkCounters* wc) {
-  __shared__ float4 s_w[kMaxSteps];  // T(x, p_i) rgb, w: height coordinate of p_i
-  __shared__ float2 s_g[kMaxSteps];  // x: elevation coordinate of v at p_i, y: t_i
-  __shared__ float s_rinv[kMaxSteps];
-  __shared__ int4 s_off[kMaxRayChunk];   // slab offsets (in units of NL x NA) of the chunk's samples
-  __shared__ float2 s_wt[kMaxRayChunk];  // elevation and height weights of the chunk's samples
-  extern __shared__ float4 s_slab[];  // Q x (NL x NA)
+  __shared__ int4 s_off[kMaxSteps];    // slab offsets (in units of NL x NA) of every sample
+  __shared__ float4 s_wt[kMaxSteps];   // their bilinear weights (h0 e0, h0 e1, h1 e0, h1 e1)
+  extern __shared__ float4 s_slab[];   // Q x (NL x NA)
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps;
   const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
   const Ray ray = make_ray(P, ih, ie);
   const double dt = ray.D / n;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    double t = (i + 0.5) * dt;
-    double h = altitude_along(P.g, ray.h0, ray.pd0, t);
-    double r = P.g.R + h;
-    float3 T = segment_transmittance(P, ray.h0, ray.pd0, t, n);
-    s_w[i] = make_float4(T.x, T.y, T.z, (float)altitude_to_index(P.g, NH, h));
-    s_g[i] = make_float2((float)elevation_to_index(P.g, NE, r, ray.pd0 + t, ray.above), (float)t);
-    s_rinv[i] = (float)(1.0 / r);
-  }
+  // this ray's samples in the per-build geometry: T(x, p_i) rgb and (height coordinate, elevation
+  // coordinate, alpha_i, beta_i)
+  const float4* __restrict__ gT = geo_T + (long long)blockIdx.x * n;
+  const float4* __restrict__ gRS = geo_rs + (long long)blockIdx.x * n;
+  const SunCoord sc = {(float)((NL - 1) * (double)P.sun_inv_den), (float)(-0.6 * 1.4426950408889634), (float)(NL - 1)};
+  const float row_bytes = (float)(NA * (int)sizeof(float4));
   // per-entry constants: heading coordinate and the terms of dot(p_i, l)
   float vl[EPT], xl[EPT];
   Lerp La[EPT];
@@ -671,41 +727,70 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
       La[q] = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
     }
   }
+  // element offsets of the four (height, elevation) slabs of every sample and their weights, shared
+  // by the workgroup
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float4 g = gRS[i];
+    const Lerp Lh = make_lerp(g.x, NH), Le = make_lerp(g.y, NE);
+    s_off[i] = make_int4((Lh.i0 * NE + Le.i0) * NLA, (Lh.i0 * NE + Le.i1) * NLA, (Lh.i1 * NE + Le.i0) * NLA,
+                         (Lh.i1 * NE + Le.i1) * NLA);
+    s_wt[i] = make_float4((1.0f - Le.s) * (1.0f - Lh.s), Le.s * (1.0f - Lh.s), (1.0f - Le.s) * Lh.s, Le.s * Lh.s);
+  }
   __syncthreads();
   for (int i0 = 0; i0 < n; i0 += Q) {
     const int qn = min(Q, n - i0);
-    if (threadIdx.x < qn) {
-      // element offsets of the four (height, elevation) slabs of sample i0 + q and their two weights,
-      // shared by the workgroup
-      const int i = i0 + threadIdx.x;
-      const Lerp Lh = make_lerp(s_w[i].w, NH), Le = make_lerp(s_g[i].x, NE);
-      s_off[threadIdx.x] = make_int4((Lh.i0 * NE + Le.i0) * NLA, (Lh.i0 * NE + Le.i1) * NLA,
-                                     (Lh.i1 * NE + Le.i0) * NLA, (Lh.i1 * NE + Le.i1) * NLA);
-      s_wt[threadIdx.x] = make_float2(Le.s, Lh.s);
-    }
-    __syncthreads();
-#pragma unroll 2
-    for (int q = 0; q < qn; ++q) {
-      const int4 o = s_off[q];
-      const float2 w = s_wt[q];
-      for (int kk = threadIdx.x; kk < NLA; kk += blockDim.x) {
-        const float3 u = mix3(ld3(dJ, (unsigned)(o.x + kk)), ld3(dJ, (unsigned)(o.y + kk)), w.x);
-        const float3 v = mix3(ld3(dJ, (unsigned)(o.z + kk)), ld3(dJ, (unsigned)(o.w + kk)), w.x);
-        const float3 c = mix3(u, v, w.y);
-        s_slab[q * NLA + kk] = make_float4(c.x, c.y, c.z, 0.f);
+    float4* slab = s_slab;
+    if (NLA <= (int)blockDim.x) {
+      // one cell per thread: consecutive samples in the same (height, elevation) cell reuse the four
+      // corner values already loaded (the offsets are workgroup-uniform, so is the branch)
+      const int kk = threadIdx.x;
+      float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, d = a;
+      int last = -1;
+      for (int q = 0; q < qn; ++q) {
+        const int4 o = s_off[i0 + q];
+        const float4 w = s_wt[i0 + q];
+        const int key = __builtin_amdgcn_readfirstlane(o.x);
+        if (key != last && kk < NLA) {
+          a = dJ[key + kk];
+          b = dJ[__builtin_amdgcn_readfirstlane(o.y) + kk];
+          c = dJ[__builtin_amdgcn_readfirstlane(o.z) + kk];
+          d = dJ[__builtin_amdgcn_readfirstlane(o.w) + kk];
+        }
+        last = key;
+        if (kk < NLA)
+          slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
+                                           fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
+                                           fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+      }
+    } else {
+      for (int q = 0; q < qn; ++q) {
+        const int4 o = s_off[i0 + q];
+        const float4 w = s_wt[i0 + q];
+        const float4* __restrict__ d0 = dJ + __builtin_amdgcn_readfirstlane(o.x);
+        const float4* __restrict__ d1 = dJ + __builtin_amdgcn_readfirstlane(o.y);
+        const float4* __restrict__ d2 = dJ + __builtin_amdgcn_readfirstlane(o.z);
+        const float4* __restrict__ d3 = dJ + __builtin_amdgcn_readfirstlane(o.w);
+        for (int kk = threadIdx.x; kk < NLA; kk += blockDim.x) {
+          const float4 a = d0[kk], b = d1[kk], c = d2[kk], d = d3[kk];
+          slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
+                                           fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
+                                           fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+        }
       }
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       if (threadIdx.x + e * blockDim.x < NLA) {
+        const float col_bytes = (float)(La[e].i0 * (int)sizeof(float4));
         // a heading coordinate with zero weight (every entry but the clamped ones) reads one column;
-        // the choice is made per wavefront (the general lookup gives the same bits for such entries),
-        // so a wavefront never runs both loops
+        // the choice is made per wavefront so that a wavefront never runs both loops
         if (__all(La[e].s == 0.0f))
-          acc[e] = ray_scatter_samples<true>(P, s_slab, s_w, s_g, s_rinv, i0, qn, NL, NA, NLA, vl[e], xl[e], La[e], acc[e]);
+          acc[e] = ray_scatter_samples<true>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
+                                             La[e], acc[e]);
         else
-          acc[e] = ray_scatter_samples<false>(P, s_slab, s_w, s_g, s_rinv, i0, qn, NL, NA, NLA, vl[e], xl[e], La[e], acc[e]);
+          acc[e] = ray_scatter_samples<false>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
+                                              La[e], acc[e]);
       }
     }
     __syncthreads();
@@ -721,8 +806,7 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
       lookups += n;
     }
   }
-  unsigned long long view = (threadIdx.x < n) ? (unsigned long long)n * ((n - threadIdx.x + blockDim.x - 1) / blockDim.x) : 0;
-  count(wc, 0, view, lookups, 0);
+  count(wc, 0, 0, lookups, 0);
 }
 
 // dJ = point-scatter over the sphere quadrature (atmosphere_lut.clj:88,91; atmosphere.clj:203-222).
@@ -807,48 +891,81 @@ __global__ void __launch_bounds__(kBlock) point_scatter_combine_kernel(const Dev
 struct DirLds {
   float4 omega;   // direction xyz, w: 1 for surface directions
   float4 weight;  // rgb quadrature weight times in-scattering
-  float4 ground;  // T(x, P) brightness / pi, w: height coordinate of P
-  float4 phat;    // P / |P|
+  float4 ground;  // T(x, P) brightness / pi, w: weight of the upper height row of P in the dE table
+  float4 phat;    // P / |P|, w: byte offset of the lower height row of P in the dE table (int bits)
 };
 
-// Sum over the quadrature directions of one entry (sun direction l, sun-elevation lerp Ls).
+// Constants of the per-direction lookups of one point-scatter launch.
+struct PsConst {
+  float hA;            // (NA - 1) / 2: heading coordinate (NA - 1) (1 + dot(omega, l)) / 2 = hA + hA dot(omega, l)
+  float na_top;        // NA - 1
+  SunCoord se;         // sun-elevation coordinate of dE (size nE[1]): alpha = -3 log2(e), beta = 0
+  unsigned e_row;      // bytes of one dE height row
+  float mie_k, mie_a, mie_b;  // Mie phase K (1 + mu^2) (a - b mu)^(-3/2): K = 3 (1 - g^2) / (8 pi (2 + g^2))
+};
+
+// phase of atmosphere.clj:56-61 with the hardware reciprocal square root (x^-3/2 = rsq(x)^3)
+__device__ __forceinline__ float mie_phase(const PsConst& pc, float mu) {
+  const float y = __builtin_amdgcn_rsqf(fmaf(-pc.mie_b, mu, pc.mie_a));
+  return pc.mie_k * fmaf(mu, mu, 1.0f) * (y * y * y);
+}
+
+__device__ __forceinline__ float4 ld4b(const float4* base, unsigned byte_offset) {
+  return *(const float4*)((const char*)base + byte_offset);
+}
+
+// Sum over the quadrature directions of one entry (sun direction l, sun-elevation lerp Ls; row_bytes:
+// byte offset of row Ls.i0 in a slab).  Heading lerp: lower column min(floor, NA - 2), so the upper
+// column is always the next one; the dE lookup likewise in both dimensions (nested as lookup2).
 template <bool kRow>
-__device__ __forceinline__ float3 point_scatter_dirs(const DevParams& P, const DirLds* s_dir, int ndirs,
+__device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const DirLds* s_dir, int ndirs,
                                                      const float4* __restrict__ sh, const float4* __restrict__ shm,
-                                                     const float4* __restrict__ dE, int NA, int NLA, Lerp Ls, float lx,
-                                                     float ly, float lz) {
+                                                     const float4* __restrict__ dE, int NA, int NLA, Lerp Ls,
+                                                     float row_bytes, float lx, float ly, float lz) {
   float3 acc = make_float3(0.f, 0.f, 0.f);
+  const unsigned d_row = (unsigned)(Ls.i1 - Ls.i0) * (unsigned)NA * (unsigned)sizeof(float4);
   for (int j = 0; j < ndirs; ++j) {
     const DirLds& dd = s_dir[j];
-    const float ol = dd.omega.x * lx + dd.omega.y * ly + dd.omega.z * lz;
-    const Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
-    const float4* row = sh + (long long)j * NLA;
-    float3 v;
-    if (kRow) {
-      row += Ls.i0 * NA;
-      v = mix3(ld3(row, La.i0), ld3(row, La.i1), La.s);
-    } else {
-      v = lookup2(row, NA, Ls, La);
+    const float ol = fmaf(dd.omega.z, lz, fmaf(dd.omega.y, ly, dd.omega.x * lx));
+    const float hc = __builtin_amdgcn_fmed3f(fmaf(pc.hA, ol, pc.hA), 0.0f, pc.na_top);
+    const float col = fminf(floorf(hc), pc.na_top - 1.0f);
+    const float sa = hc - col;
+    const unsigned off = (unsigned)fmaf(col, (float)sizeof(float4), row_bytes);
+    const float4* row = sh + j * NLA;
+    float3 v = lerp3(ld4b(row, off), ld4b(row, off + 16u), sa);
+    if (!kRow) {
+      const float3 u = lerp3(ld4b(row, off + d_row), ld4b(row, off + d_row + 16u), sa);
+      v = make_float3(fmaf(Ls.s, u.x - v.x, v.x), fmaf(Ls.s, u.y - v.y, v.y), fmaf(Ls.s, u.z - v.z, v.z));
     }
     if (shm) {
-      const float4* rowm = shm + (long long)j * NLA;
-      float3 m;
-      if (kRow) {
-        rowm += Ls.i0 * NA;
-        m = mix3(ld3(rowm, La.i0), ld3(rowm, La.i1), La.s);
-      } else {
-        m = lookup2(rowm, NA, Ls, La);
+      const float4* rowm = shm + j * NLA;
+      float3 m = lerp3(ld4b(rowm, off), ld4b(rowm, off + 16u), sa);
+      if (!kRow) {
+        const float3 u = lerp3(ld4b(rowm, off + d_row), ld4b(rowm, off + d_row + 16u), sa);
+        m = make_float3(fmaf(Ls.s, u.x - m.x, m.x), fmaf(Ls.s, u.y - m.y, m.y), fmaf(Ls.s, u.z - m.z, m.z));
       }
-      const float ph = phase_f(P.g_mie, ol);
-      v.x += m.x * ph; v.y += m.y * ph; v.z += m.z * ph;
+      const float ph = mie_phase(pc, ol);
+      v.x = fmaf(m.x, ph, v.x);
+      v.y = fmaf(m.y, ph, v.y);
+      v.z = fmaf(m.z, ph, v.z);
     }
     if (dd.omega.w != 0.f) {
-      const float spl = dd.phat.x * lx + dd.phat.y * ly + dd.phat.z * lz;
-      const float3 e = lookup2(dE, P.nE[1], make_lerp(dd.ground.w, P.nE[0]),
-                               make_lerp(sun_elevation_coord(P, P.nE[1], spl), P.nE[1]));
-      v.x += dd.ground.x * e.x; v.y += dd.ground.y * e.y; v.z += dd.ground.z * e.z;
+      const float spl = fmaf(dd.phat.z, lz, fmaf(dd.phat.y, ly, dd.phat.x * lx));
+      const float ce = __builtin_amdgcn_fmed3f(
+          fmaf(-pc.se.A, fexp2(fmaf(-3.0f * 1.4426950408889634f, spl, pc.se.c)), pc.se.A), 0.0f, pc.se.top);
+      const float ecol = fminf(floorf(ce), pc.se.top - 1.0f);
+      const float se = ce - ecol;
+      const unsigned eo = (unsigned)__float_as_int(dd.phat.w) + (unsigned)ecol * (unsigned)sizeof(float4);
+      const float3 u = lerp3(ld4b(dE, eo), ld4b(dE, eo + 16u), se);
+      const float3 t = lerp3(ld4b(dE, eo + pc.e_row), ld4b(dE, eo + pc.e_row + 16u), se);
+      const float wh = dd.ground.w;
+      v.x = fmaf(dd.ground.x, fmaf(wh, t.x - u.x, u.x), v.x);
+      v.y = fmaf(dd.ground.y, fmaf(wh, t.y - u.y, u.y), v.y);
+      v.z = fmaf(dd.ground.z, fmaf(wh, t.z - u.z, u.z), v.z);
     }
-    acc.x += dd.weight.x * v.x; acc.y += dd.weight.y * v.y; acc.z += dd.weight.z * v.z;
+    acc.x = fmaf(dd.weight.x, v.x, acc.x);
+    acc.y = fmaf(dd.weight.y, v.y, acc.y);
+    acc.z = fmaf(dd.weight.z, v.z, acc.z);
   }
   return acc;
 }
@@ -869,6 +986,7 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   index_to_elevation(P.g, NE, r0, ie, se, ce, above_unused);
   const float bm = (float)exp(-h0 * P.inv_scale_mie), br = (float)exp(-h0 * P.inv_scale_ray);
   const DirGeom* gh = geom + (long long)ih * ndirs;
+  const int nE0 = P.nE[0], nE1 = P.nE[1];
   for (int j = threadIdx.x; j < ndirs; j += blockDim.x) {
     Dir d = dirs[j];
     DirGeom g = gh[j];
@@ -879,10 +997,22 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
     dl.omega = make_float4(g.omega.x, g.omega.y, g.omega.z, g.ground.w);
     dl.weight = make_float4((P.sca_mie[0] * pm + P.sca_ray[0] * pr) * w, (P.sca_mie[1] * pm + P.sca_ray[1] * pr) * w,
                             (P.sca_mie[2] * pm + P.sca_ray[2] * pr) * w, 0.f);
-    dl.ground = make_float4(g.ground.x, g.ground.y, g.ground.z, g.phat.w);
-    dl.phat = g.phat;
+    // height lerp of P in the dE table (interpolate.clj:75-98), lower row min(floor, nE0 - 2)
+    const float hc = fminf(fmaxf(g.phat.w, 0.0f), (float)(nE0 - 1));
+    const float hrow = fminf(floorf(hc), (float)(nE0 - 2));
+    dl.ground = make_float4(g.ground.x, g.ground.y, g.ground.z, hc - hrow);
+    dl.phat = make_float4(g.phat.x, g.phat.y, g.phat.z, __int_as_float((int)hrow * nE1 * (int)sizeof(float4)));
     s_dir[j] = dl;
   }
+  const float gm = P.g_mie;
+  PsConst pc;
+  pc.hA = 0.5f * (float)(NA - 1);
+  pc.na_top = (float)(NA - 1);
+  pc.se = SunCoord{(float)((nE1 - 1) * (double)P.sun_inv_den), (float)(-0.6 * 1.4426950408889634), (float)(nE1 - 1)};
+  pc.e_row = (unsigned)nE1 * (unsigned)sizeof(float4);
+  pc.mie_k = 3.0f * (1.0f - gm * gm) / (8.0f * 3.14159265358979f * (2.0f + gm * gm));
+  pc.mie_a = 1.0f + gm * gm;
+  pc.mie_b = 2.0f * gm;
   __syncthreads();
   const float4* sh = slab + (long long)ih * ndirs * NLA;
   const float4* shm = slab_mie ? slab_mie + (long long)ih * ndirs * NLA : nullptr;
@@ -895,10 +1025,12 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
     index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2d);
     const float lx = (float)l0, ly = (float)l1, lz = (float)l2d;
     const Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
+    const float row_bytes = (float)(Ls.i0 * NA * (int)sizeof(float4));
     // a grid entry's own sun-elevation coordinate is an integer, so its weight is exactly zero and only
-    // one row of each slab is read (lookup2_skip, bit for bit); the general lookup stays for safety
-    const float3 acc = Ls.s == 0.0f ? point_scatter_dirs<true>(P, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, lx, ly, lz)
-                                    : point_scatter_dirs<false>(P, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, lx, ly, lz);
+    // one row of each slab is read; the general lookup stays for safety
+    const float3 acc = Ls.s == 0.0f
+                           ? point_scatter_dirs<true>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz)
+                           : point_scatter_dirs<false>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz);
     l4 += (unsigned long long)ndirs * (shm ? 2 : 1);
     if (wc)
       for (int j = 0; j < ndirs; ++j) l2 += s_dir[j].omega.w != 0.f;
@@ -1147,16 +1279,25 @@ hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCou
   return hipGetLastError();
 }
 
+hipError_t launch_ray_geometry(const DevParams& P, WorkCounters* wc, hipStream_t stream) {
+  if (!P.ray_geo) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ray_geometry_kernel, dim3(blocks_for((long long)ray_geo_samples(P))), dim3(kBlock), 0, stream, P,
+                     wc);
+  return hipGetLastError();
+}
+
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
                                  hipStream_t stream) {
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   if (P.sun_rows > 0) {
-    if (!P.sun_depth || P.sun_rows < 2 || P.sun_cols < 2) return hipErrorInvalidValue;
+    if (!P.sun_depth || !P.ray_geo || P.sun_rows < 2 || P.sun_cols < 2) return hipErrorInvalidValue;
     hipLaunchKernelGGL(sun_depth_kernel, dim3(blocks_for((long long)P.sun_rows * P.sun_cols)), dim3(kBlock), 0, stream, P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    size_t shmem = (size_t)P.ray_steps * (2 * sizeof(double) + 2 * sizeof(float4) + sizeof(float2));
-    hipLaunchKernelGGL(single_scatter_table_kernel, dim3(rays), dim3(kSSTBlock), shmem, stream, P, rayleigh, mie, wc);
+    const RayGeoView V = ray_geo_view(P);
+    const size_t shmem = (size_t)P.ray_steps * (sizeof(double2) + 2 * sizeof(float4) + sizeof(float2));
+    hipLaunchKernelGGL(single_scatter_table_kernel, dim3(rays), dim3(kSSTBlock), shmem, stream, P, V.th, V.T, V.ss,
+                       V.row, rayleigh, mie, wc);
     return hipGetLastError();
   }
   unsigned groups = (unsigned)((P.nS[2] * P.nS[3] + kSSWave - 1) / kSSWave);
@@ -1194,6 +1335,8 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
 
 // Samples per folded-slab chunk of the ray-scatter kernel: about 2048 slab entries (32 KB of LDS) by
 // default; SFSIM_ATMO_RAY_CHUNK (1..8) overrides it.
+// Samples per chunk of the ray-scatter kernel: its folded slabs take Q x NL x NA float4 of LDS, kept
+// within 32 KiB (plus 8 KiB of per-sample offsets and weights).
 int ray_scatter_chunk(const DevParams& P) {
   int nla = P.nS[2] * P.nS[3];
   int q = 2048 / nla;
@@ -1201,21 +1344,24 @@ int ray_scatter_chunk(const DevParams& P) {
     int v = std::atoi(env);
     if (v >= 1) q = v;
   }
-  return q < 1 ? 1 : (q > kMaxRayChunk ? kMaxRayChunk : q);
+  const int qmax = std::min(kMaxRayChunk, 2048 / nla);
+  return q < 1 ? 1 : (q > qmax ? std::max(qmax, 1) : q);
 }
 
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream) {
+  if (!P.ray_geo) return hipErrorInvalidValue;
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   int nla = P.nS[2] * P.nS[3];
   int Q = ray_scatter_chunk(P);
   size_t shmem = sizeof(float4) * (size_t)Q * nla;
+  const RayGeoView V = ray_geo_view(P);
   if (nla <= kBlock)
-    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
   else if (nla <= 2 * kBlock)
-    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
   else if (nla <= 4 * kBlock)
-    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

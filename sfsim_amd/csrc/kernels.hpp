@@ -10,6 +10,9 @@ namespace sfsim {
 
 hipError_t launch_transmittance(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream);
 hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream);
+// Per-(view ray, sample) geometry into P.ray_geo (once per build, before single and ray scatter).
+hipError_t launch_ray_geometry(const DevParams& P, WorkCounters* wc, hipStream_t stream);
+// R1 and M1; with a sun table (P.sun_rows > 0) builds it first and reads P.ray_geo.
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
                                  hipStream_t stream);
 // Per-(height, direction) geometry of the sphere quadrature (once per build); geom holds
@@ -23,6 +26,7 @@ size_t direction_geometry_bytes(const DevParams& P, int ndirs);
 hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
                                 const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
                                 WorkCounters* wc, hipStream_t stream);
+// Reads P.ray_geo (launch_ray_geometry).
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream);
 hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
