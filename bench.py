@@ -9,8 +9,8 @@ out is not part of a step).  Per-kernel durations come from the same number of k
 builds with HIP events (sfsim_atmo_build_timed) after the timed steps.
 
 Prints one JSON line with the throughput (LUT entries/s), the roofline of the dominant kernel
-(single scattering, measured with HIP events on the library's stream) and the FP64 CPU oracle timed
-on a bounded sample of the same workload on this host.
+(ray scattering, five launches per build, measured with HIP events on the library's stream) and the
+FP64 CPU oracle timed on a bounded sample of the same workload on this host.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
 """
@@ -28,9 +28,8 @@ sys.path.insert(0, ROOT)
 METRIC = "LUT entries/s (5-order multi-scatter, 1 GPU) at ≤1e-3 rel err vs FP64 oracle"
 WORKLOAD = "full-5-order"
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
-OPS_PER_SAMPLE = 20       # extinction sample: position 3, |p|^2 3, sqrt 1, -R 1, 2 scales, 2 exp, 2x RGB accumulate 6
-OPS_PER_LOOKUP4 = 136     # 16 corners x 3 channels x 2 + 40 index operations
-OPS_PER_LOOKUP2 = 20
+OPS_PER_LOOKUP4 = 136     # 4D multilinear RGB lookup: 15 lerps x 3 channels x 2 + 46 coordinate/index operations
+OPS_PER_RAY_SAMPLE = OPS_PER_LOOKUP4 + 6  # ray scatter: one dJ lookup and T(x, p_i) * value accumulated (3 FMA)
 
 
 def parse():
@@ -171,20 +170,22 @@ def main():
     value = n_entries / (ms_per_step / 1e3)
     errors = check_against_fixture(atm, ctx, P, args.workload)
 
-    # roofline of the dominant kernel: single scattering (counted work / HIP-event duration)
+    # roofline of the dominant kernel: ray scattering (algorithmic work / HIP-event duration per launch).
+    # One launch evaluates every (entry, ray sample) pair once: scatter entries x ray_steps lookups.
     work = ctx.count_work(P)
-    ss_ms, ss_n = kernel_ms["single_scatter"]
-    ss_avg_ms = ss_ms / max(ss_n, 1)
-    ss_flops = OPS_PER_SAMPLE * work["single_scatter_samples"]
-    achieved = ss_flops / (ss_avg_ms * 1e-3) / 1e12
+    rs_ms, rs_n = kernel_ms["ray_scatter"]
+    rs_avg_ms = rs_ms / max(rs_n, 1)
+    rs_samples = int(np.prod(P.scatter_shape)) * P.ray_steps
+    rs_flops = OPS_PER_RAY_SAMPLE * rs_samples
+    achieved = rs_flops / (rs_avg_ms * 1e-3) / 1e12
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_single_scatter_full-5-order.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_ray_scatter_full-5-order.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
-    roofline = {"bound": "valu", "kernel": "single_scatter", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+    roofline = {"bound": "valu", "kernel": "ray_scatter", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                "avg_launch_ms": round(ss_avg_ms, 4), "algorithmic_flops": ss_flops,
-                "work": work}
+                "avg_launch_ms": round(rs_avg_ms, 4), "algorithmic_flops": rs_flops,
+                "samples_per_launch": rs_samples, "work": work}
 
     seeded = None
     if not args.no_seeded and P.iterations == 5:
