@@ -264,6 +264,17 @@ __device__ __forceinline__ void count(WorkCounters* wc, unsigned long long singl
   }
 }
 
+// XCD-aware order of the view rays: workgroups are dispatched to the 8 XCDs round robin (workgroup b
+// to XCD b mod 8), each with its own L2.  Giving XCD x the contiguous range of rays
+// [x rays / 8, (x + 1) rays / 8) (4 of 32 heights at the full workload) keeps the tables that a height
+// range reads (dJ cells, point-scatter slabs, sun-table rows) in fewer L2s.  A bijection when rays is a
+// multiple of 8; identity otherwise.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_ray(int b, int rays) {
+  if (rays % kXcds != 0) return b;
+  return (b % kXcds) * (rays / kXcds) + b / kXcds;
+}
+
 // ------------------------------------------------------------------------------ kernels
 // T table: atmosphere_lut.clj:74 over transmittance-space (atmosphere.clj:291-319)
 __global__ void __launch_bounds__(kBlock) transmittance_kernel(const DevParams P, float4* __restrict__ out,
@@ -561,7 +572,7 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
   const int NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps, cols = P.sun_cols;
-  const int rayid = blockIdx.x;
+  const int rayid = xcd_ray(blockIdx.x, P.nS[0] * NE);
   const int ih = rayid / NE, ie = rayid - ih * NE;
   const Ray ray = make_ray(P, ih, ie);
   const double dt = ray.D / n;
@@ -697,13 +708,14 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps;
-  const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
+  const int rayid = xcd_ray(blockIdx.x, NH * NE);
+  const int ih = rayid / NE, ie = rayid - ih * NE;
   const Ray ray = make_ray(P, ih, ie);
   const double dt = ray.D / n;
   // this ray's samples in the per-build geometry: T(x, p_i) rgb and (height coordinate, elevation
   // coordinate, alpha_i, beta_i)
-  const float4* __restrict__ gT = geo_T + (long long)blockIdx.x * n;
-  const float4* __restrict__ gRS = geo_rs + (long long)blockIdx.x * n;
+  const float4* __restrict__ gT = geo_T + (long long)rayid * n;
+  const float4* __restrict__ gRS = geo_rs + (long long)rayid * n;
   const SunCoord sc = {(float)((NL - 1) * (double)P.sun_inv_den), (float)(-0.6 * 1.4426950408889634), (float)(NL - 1)};
   const float row_bytes = (float)(NA * (int)sizeof(float4));
   // per-entry constants: heading coordinate and the terms of dot(p_i, l)
@@ -795,7 +807,7 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
     }
     __syncthreads();
   }
-  const long long base = (long long)blockIdx.x * NLA;
+  const long long base = (long long)rayid * NLA;
   const float a = (float)dt;
   unsigned long long lookups = 0;
 #pragma unroll
@@ -866,15 +878,16 @@ __global__ void __launch_bounds__(kBlock) point_scatter_combine_kernel(const Dev
                                                                        float4* __restrict__ slab_mie) {
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
-  const int ih = blockIdx.x / ndirs;
+  const int b = xcd_ray(blockIdx.x, NH * ndirs);  // the XCD that reads this height's slabs (same ranges)
+  const int ih = b / ndirs;
   const double r0 = index_to_radius(P.g, NH, ih);
   const Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
-  const Lerp Le = make_lerp(geom[blockIdx.x].omega.w, NE);
+  const Lerp Le = make_lerp(geom[b].omega.w, NE);
   const long long sh = (long long)NE * NLA;
   const long long o00 = Lh.i0 * sh + (long long)Le.i0 * NLA, o01 = Lh.i0 * sh + (long long)Le.i1 * NLA;
   const long long o10 = Lh.i1 * sh + (long long)Le.i0 * NLA, o11 = Lh.i1 * sh + (long long)Le.i1 * NLA;
-  float4* dst = slab + (long long)blockIdx.x * NLA;
-  float4* dstm = slab_mie ? slab_mie + (long long)blockIdx.x * NLA : nullptr;
+  float4* dst = slab + (long long)b * NLA;
+  float4* dstm = slab_mie ? slab_mie + (long long)b * NLA : nullptr;
   for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
     float3 c = mix3(mix3(ld3(dS, o00 + k), ld3(dS, o01 + k), Le.s), mix3(ld3(dS, o10 + k), ld3(dS, o11 + k), Le.s), Lh.s);
     dst[k] = make_float4(c.x, c.y, c.z, 0.f);
@@ -979,7 +992,8 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   extern __shared__ DirLds s_dir[];
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
-  const int ih = blockIdx.x / NE, ie = blockIdx.x - ih * NE;
+  const int rayid = xcd_ray(blockIdx.x, NH * NE);
+  const int ih = rayid / NE, ie = rayid - ih * NE;
   const double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
   double se, ce;
   bool above_unused;
@@ -1016,7 +1030,7 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   __syncthreads();
   const float4* sh = slab + (long long)ih * ndirs * NLA;
   const float4* shm = slab_mie ? slab_mie + (long long)ih * ndirs * NLA : nullptr;
-  const long long base = (long long)blockIdx.x * NLA;
+  const long long base = (long long)rayid * NLA;
   unsigned long long l4 = 0, l2 = 0;
   for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
     int is = k / NA, ia = k - is * NA;
