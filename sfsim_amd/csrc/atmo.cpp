@@ -70,6 +70,7 @@ struct sfsim_atmo_ctx {
   int sun_rows = kSunRows, sun_cols = kSunCols;
   DeviceBuffer sun_depth;
   DeviceBuffer ray_geo;  // per-(view ray, sample) geometry of single and ray scatter
+  DeviceBuffer rmap;     // per-entry 4D re-sampling lookups of the pipelines
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
   Surrogate sg;
@@ -296,6 +297,7 @@ DevParams dev_params(const sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
     P.sun_depth = (float2*)ctx->sun_depth.ptr;
   }
   P.ray_geo = (char*)ctx->ray_geo.ptr;
+  P.rmap = (char*)ctx->rmap.ptr;  // filled by launch_resample_map at the start of a pipeline
   return P;
 }
 
@@ -307,6 +309,7 @@ int ensure_sun_depth(sfsim_atmo_ctx* ctx) {
 // The sun table and the per-(view ray, sample) geometry buffer of the single/ray-scatter kernels.
 int ensure_scatter_buffers(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(ensure_sun_depth(ctx));
+  TRY(ensure(ctx, ctx->rmap, resample_map_bytes(make_dev_params(p))));
   return ensure(ctx, ctx->ray_geo, ray_geometry_bytes(make_dev_params(p)));
 }
 
@@ -482,6 +485,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), wc, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), wc, ctx->stream));
   LAUNCH(timer, kRayGeometry, launch_ray_geometry(P, wc, ctx->stream));
+  LAUNCH(timer, kAccumulate, launch_resample_map(P, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream));
   if (p->iterations > 0)
     LAUNCH(timer, kDirectionGeometry,
@@ -668,6 +672,7 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), nullptr, ctx->stream));
   LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), nullptr, ctx->stream));
   LAUNCH(timer, kRayGeometry, launch_ray_geometry(P, nullptr, ctx->stream));
+  LAUNCH(timer, kAccumulate, launch_resample_map(P, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), nullptr, ctx->stream));
   LAUNCH(timer, kDirectionGeometry,
          launch_direction_geometry(P, sdirs, ctx->sphere_n, ctx->geom.ptr, nullptr, ctx->stream));
@@ -881,7 +886,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
@@ -1022,6 +1027,7 @@ int sfsim_atmo_resample_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, i
   if (space != 2 && space != 3 && space != 4) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "space must be 2, 3 or 4");
   if (!a || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "a and out must not be NULL");
   DevParams P = dev_params(ctx, p);
+  P.rmap = nullptr;  // a single re-sampling computes its lookups itself
   size_t n = space == 4   ? (size_t)count4(p->scatter_shape)
              : space == 2 ? (size_t)count2(p->transmittance_shape)
                           : (size_t)count2(p->surface_radiance_shape);

@@ -45,7 +45,14 @@ struct DevParams {
   // per-(view ray, sample) geometry of the single-scatter and ray-scatter kernels, computed once per
   // build (ray_geometry_bytes(P) bytes, laid out by ray_geo_view)
   char* ray_geo;
+  // per-entry lookup of the 4D re-sampling (forward(backward(i))): NH x NE x NL x NA float4 weights
+  // followed by as many ushort4 lower indices (resample_map_bytes); null: computed per launch
+  char* rmap;
 };
+
+__host__ __device__ inline size_t resample_map_bytes(const DevParams& P) {
+  return (size_t)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3] * (sizeof(float4) + sizeof(ushort4));
+}
 
 // Views of DevParams::ray_geo: m = NH x NE x ray_steps samples, sample i of view ray (ih, ie) at
 // (ih NE + ie) ray_steps + i; t_i = (i + 1/2) D / ray_steps along the ray from the grid point.

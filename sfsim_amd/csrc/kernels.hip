@@ -1103,6 +1103,49 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParam
 }
 
 // f(backward(i)) re-sampling of interpolated tables (interpolate.clj:54-72 with interpolation-table).
+// Lerps of forward(backward(i)) for 4D entry i = ((ih NE + ie) NL + is) NA + ia (atmosphere.clj:233-422).
+__device__ __forceinline__ void resample4_lerps(const DevParams& P, int ih, int ie, int is, int ia, Lerp& Lh, Lerp& Le,
+                                                Lerp& Ls, Lerp& La) {
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  double r0 = index_to_radius(P.g, NH, ih);
+  double se, ce;
+  bool above;
+  index_to_elevation(P.g, NE, r0, ie, se, ce, above);
+  double sin_s = index_to_sin_sun_elevation(NL, is);
+  double l0, l1, l2;
+  index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2);
+  Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
+  Le = make_lerp((float)elevation_to_index(P.g, NE, r0, r0 * se, above), NE);
+  Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
+  La = make_lerp((float)sun_angle_to_index(NA, se * l0 + ce * l1), NA);
+}
+
+__device__ __forceinline__ Lerp lerp_of(int i0, float s, int N) {
+  Lerp L;
+  L.i0 = i0;
+  L.i1 = min(i0 + 1, N - 1);
+  L.s = s;
+  return L;
+}
+
+// The 4D re-sampling lookups of a build, once (every accumulation and final table reads them).
+__global__ void __launch_bounds__(kBlock) resample_map_kernel(const DevParams P) {
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const long long n = (long long)NH * NE * NL * NA;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int ia = (int)(idx % NA);
+  long long q = idx / NA;
+  const int is = (int)(q % NL);
+  q /= NL;
+  const int ie = (int)(q % NE), ih = (int)(q / NE);
+  Lerp Lh, Le, Ls, La;
+  resample4_lerps(P, ih, ie, is, ia, Lh, Le, Ls, La);
+  ((float4*)P.rmap)[idx] = make_float4(Lh.s, Le.s, Ls.s, La.s);
+  ((ushort4*)(P.rmap + n * sizeof(float4)))[idx] =
+      make_ushort4((unsigned short)Lh.i0, (unsigned short)Le.i0, (unsigned short)Ls.i0, (unsigned short)La.i0);
+}
+
 __global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int space, const float4* __restrict__ a,
                                                           const float4* __restrict__ b, float4* __restrict__ out4,
                                                           float* __restrict__ out_rgb, int tiled) {
@@ -1111,24 +1154,28 @@ __global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int
   long long dst;
   if (space == 4) {
     const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
-    if (idx >= (long long)NH * NE * NL * NA) return;
-    int ia = (int)(idx % NA);
-    long long q = idx / NA;
-    int is = (int)(q % NL);
-    q /= NL;
-    int ie = (int)(q % NE);
-    int ih = (int)(q / NE);
-    double r0 = index_to_radius(P.g, NH, ih);
-    double se, ce;
-    bool above;
-    index_to_elevation(P.g, NE, r0, ie, se, ce, above);
-    double sin_s = index_to_sin_sun_elevation(NL, is);
-    double l0, l1, l2;
-    index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2);
-    Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
-    Lerp Le = make_lerp((float)elevation_to_index(P.g, NE, r0, r0 * se, above), NE);
-    Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
-    Lerp La = make_lerp((float)sun_angle_to_index(NA, se * l0 + ce * l1), NA);
+    const long long n = (long long)NH * NE * NL * NA;
+    if (idx >= n) return;
+    Lerp Lh, Le, Ls, La;
+    int ih = 0, ie = 0, is = 0, ia = 0;
+    if (!P.rmap || tiled) {
+      ia = (int)(idx % NA);
+      long long q = idx / NA;
+      is = (int)(q % NL);
+      q /= NL;
+      ie = (int)(q % NE);
+      ih = (int)(q / NE);
+    }
+    if (P.rmap) {
+      const float4 w = ((const float4*)P.rmap)[idx];
+      const ushort4 i = ((const ushort4*)(P.rmap + n * sizeof(float4)))[idx];
+      Lh = lerp_of(i.x, w.x, NH);
+      Le = lerp_of(i.y, w.y, NE);
+      Ls = lerp_of(i.z, w.z, NL);
+      La = lerp_of(i.w, w.w, NA);
+    } else {
+      resample4_lerps(P, ih, ie, is, ia, Lh, Le, Ls, La);
+    }
     if (Lh.s == 0.0f && Le.s == 0.0f && Ls.s == 0.0f && La.s == 0.0f) {
       // the entry maps onto a grid point (every weight zero): the lookup is that point's value, bit for bit
       const long long c = (((long long)Lh.i0 * NE + Le.i0) * NL + Ls.i0) * NA + La.i0;
@@ -1386,6 +1433,13 @@ hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndir
   unsigned entries = (unsigned)(P.nE[0] * P.nE[1]);
   hipLaunchKernelGGL(surface_radiance_kernel, dim3(entries), dim3(kBlock), 0, stream, P, dirs, ndirs, dS, mie, out,
                      wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_resample_map(const DevParams& P, hipStream_t stream) {
+  if (!P.rmap) return hipErrorInvalidValue;
+  const long long n = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
+  hipLaunchKernelGGL(resample_map_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P);
   return hipGetLastError();
 }
 

@@ -34,6 +34,9 @@ hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndir
 // Re-sample a (+ b when non-null) at forward(backward(i)) of the space (2, 3 or 4).  Output either as
 // float4 grid-order table (out4) or as packed RGB floats (out_rgb): grid order, or for space 4 with
 // tiled != 0 the 4D->2D tiling of the .scatter files.
+// The 4D re-sampling lookups of every entry into P.rmap (once per build, before any space-4 resample
+// that runs with P.rmap set).
+hipError_t launch_resample_map(const DevParams& P, hipStream_t stream);
 hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
                            float* out_rgb, int tiled, hipStream_t stream);
 // c = a + b (float4 tables)
