@@ -71,6 +71,7 @@ struct sfsim_atmo_ctx {
   DeviceBuffer sun_depth;
   DeviceBuffer ray_geo;  // per-(view ray, sample) geometry of single and ray scatter
   DeviceBuffer rmap;     // per-entry 4D re-sampling lookups of the pipelines
+  DeviceBuffer sgeo;     // half-sphere direction geometry per surface-radiance height
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
   Surrogate sg;
@@ -298,7 +299,13 @@ DevParams dev_params(const sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   }
   P.ray_geo = (char*)ctx->ray_geo.ptr;
   P.rmap = (char*)ctx->rmap.ptr;  // filled by launch_resample_map at the start of a pipeline
+  P.sgeo = (float4*)ctx->sgeo.ptr;  // filled by launch_surface_geometry before the surface-radiance stages
   return P;
+}
+
+// The surface-radiance direction geometry buffer (after ensure_dirs for the half sphere).
+int ensure_surface_geometry(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  return ensure(ctx, ctx->sgeo, (size_t)p->surface_radiance_shape[0] * ctx->half_n * sizeof(float4));
 }
 
 int ensure_sun_depth(sfsim_atmo_ctx* ctx) {
@@ -441,6 +448,7 @@ int prepare_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
     TRY(validate_half_sphere(ctx, p->ray_steps));
     TRY(ensure_dirs(ctx, 0, p->sphere_steps));
     TRY(ensure_dirs(ctx, 1, p->ray_steps));
+    TRY(ensure_surface_geometry(ctx, p));
   }
   const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
   const size_t nS = (size_t)count4(p->scatter_shape);
@@ -487,9 +495,11 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
   LAUNCH(timer, kRayGeometry, launch_ray_geometry(P, wc, ctx->stream));
   LAUNCH(timer, kAccumulate, launch_resample_map(P, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream));
-  if (p->iterations > 0)
+  if (p->iterations > 0) {
     LAUNCH(timer, kDirectionGeometry,
            launch_direction_geometry(P, sdirs, ctx->sphere_n, ctx->geom.ptr, wc, ctx->stream));
+    LAUNCH(timer, kDirectionGeometry, launch_surface_geometry(P, hdirs, ctx->half_n, ctx->stream));
+  }
   // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
   const float4* S_cur = f4(ctx->R1);
   const float4* E_cur = nullptr;
@@ -504,7 +514,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
                                 f4(ctx->slab_mie), f4(ctx->dJ), wc, ctx->stream));
     const int dE_new = 1 - dE_cur;
     LAUNCH(timer, kSurfaceRadiance,
-           launch_surface_radiance(P, hdirs, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
+           launch_surface_radiance(P, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
     const int dS_new = dS_cur < 0 ? 0 : 1 - dS_cur;
     LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[dS_new]), wc, ctx->stream));
     dE_cur = dE_new;
@@ -626,6 +636,7 @@ int prepare_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, const
   TRY(validate_half_sphere(ctx, p->ray_steps));
   TRY(ensure_dirs(ctx, 0, p->sphere_steps));
   TRY(ensure_dirs(ctx, 1, p->ray_steps));
+  TRY(ensure_surface_geometry(ctx, p));
   TRY(ensure_scatter_buffers(ctx, p));
   const DevParams P = dev_params(ctx, p);
   const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
@@ -676,6 +687,7 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), nullptr, ctx->stream));
   LAUNCH(timer, kDirectionGeometry,
          launch_direction_geometry(P, sdirs, ctx->sphere_n, ctx->geom.ptr, nullptr, ctx->stream));
+  LAUNCH(timer, kDirectionGeometry, launch_surface_geometry(P, hdirs, ctx->half_n, ctx->stream));
   // the network's order sums: P_1..P_4 into pred[0..3] (the ones below the seed feed the final
   // table's re-sampling chain), its P_5 into dS[1] (residual)
   float4* want[kMlpOrders];
@@ -701,10 +713,10 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   if (j >= 2) {
     LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), f4(ctx->pred[j - 2]), B, n4, ctx->stream));
     LAUNCH(timer, kSurfaceRadiance,
-           launch_surface_radiance(P, hdirs, ctx->half_n, B, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
+           launch_surface_radiance(P, ctx->half_n, B, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
   } else {
-    LAUNCH(timer, kSurfaceRadiance, launch_surface_radiance(P, hdirs, ctx->half_n, f4(ctx->R1), f4(ctx->M1),
-                                                            f4(ctx->dE[1]), nullptr, ctx->stream));
+    LAUNCH(timer, kSurfaceRadiance, launch_surface_radiance(P, ctx->half_n, f4(ctx->R1), f4(ctx->M1), f4(ctx->dE[1]),
+                                                            nullptr, ctx->stream));
   }
   LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->dE[0]), f4(ctx->dE[1]), Esum, e4, ctx->stream));
   // m applications of X <- x_0 + L X: application a leaves the computed P_{j+1+a} in pred[j + a]; after
@@ -716,7 +728,7 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
                                 f4(ctx->slab_mie), f4(ctx->dJ), nullptr, ctx->stream));
     LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), Pk, nullptr, ctx->stream));
     LAUNCH(timer, kSurfaceRadiance,
-           launch_surface_radiance(P, hdirs, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
+           launch_surface_radiance(P, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
     if (a + 1 < m) {
       LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), Pk, A, n4, ctx->stream));
       LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->dE[0]), f4(ctx->dE[1]), Esum, e4, ctx->stream));
@@ -886,7 +898,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
@@ -1007,6 +1019,7 @@ int sfsim_atmo_surface_radiance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_para
   TRY(validate_half_sphere(ctx, p->ray_steps));
   if (!dS || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dS and out must not be NULL");
   TRY(ensure_dirs(ctx, 1, p->ray_steps));
+  TRY(ensure_surface_geometry(ctx, p));
   DevParams P = dev_params(ctx, p);
   size_t n = (size_t)count4(p->scatter_shape), ne = (size_t)count2(p->surface_radiance_shape);
   const float4 *ddS, *dM;
@@ -1015,8 +1028,8 @@ int sfsim_atmo_surface_radiance_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_para
   int rc;
   float4* o = stage_out(ctx, ne, &rc);
   TRY(rc);
-  HIP_TRY(ctx, launch_surface_radiance(P, (const Dir*)ctx->half_dirs.ptr, ctx->half_n, ddS, dM, o, nullptr,
-                                       ctx->stream));
+  HIP_TRY(ctx, launch_surface_geometry(P, (const Dir*)ctx->half_dirs.ptr, ctx->half_n, ctx->stream));
+  HIP_TRY(ctx, launch_surface_radiance(P, ctx->half_n, ddS, dM, o, nullptr, ctx->stream));
   return download_table(ctx, o, out, ne);
 }
 

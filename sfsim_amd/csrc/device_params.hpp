@@ -48,6 +48,9 @@ struct DevParams {
   // per-entry lookup of the 4D re-sampling (forward(backward(i))): NH x NE x NL x NA float4 weights
   // followed by as many ushort4 lower indices (resample_map_bytes); null: computed per launch
   char* rmap;
+  // per (surface-radiance height, half-sphere direction): omega.x, omega.y, elevation coordinate of
+  // omega at the height's grid point, cosine-weighted quadrature weight (nE[0] x directions float4)
+  float4* sgeo;
 };
 
 __host__ __device__ inline size_t resample_map_bytes(const DevParams& P) {

@@ -1055,8 +1055,20 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
 
 // dE' = surface-radiance of dS (atmosphere_lut.clj:89,93; atmosphere.clj:225-230): one workgroup per
 // entry of the surface-radiance table, lanes over half-sphere directions, workgroup reduction.
-__global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P, const Dir* __restrict__ dirs,
-                                                                  int ndirs, const float4* __restrict__ dS,
+// The half-sphere directions' geometry at every surface-radiance height (once per build).
+__global__ void __launch_bounds__(kBlock) surface_geometry_kernel(const DevParams P, const Dir* __restrict__ dirs,
+                                                                  int ndirs) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P.nE[0] * ndirs) return;
+  const int ih = idx / ndirs, j = idx - ih * ndirs;
+  const double r0 = index_to_radius(P.g, P.nE[0], ih);
+  const Dir d = dirs[j];
+  P.sgeo[idx] = make_float4((float)d.x, (float)d.y, (float)elevation_to_index(P.g, P.nS[1], r0, r0 * d.x, true),
+                            (float)(d.x * d.w));  // (omega . normal) * quadrature weight, normal = (1 0 0)
+}
+
+__global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P, int ndirs,
+                                                                  const float4* __restrict__ dS,
                                                                   const float4* __restrict__ mie,
                                                                   float4* __restrict__ out, WorkCounters* wc) {
   __shared__ float3 s_red[kBlock / 64];
@@ -1068,20 +1080,28 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParam
   const float lx = (float)s, ly = (float)c;
   const Lerp Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
   const Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * s) / r0), NL);
+  const float gm = P.g_mie;
+  const float mk = 3.0f * (1.0f - gm * gm) / (8.0f * 3.14159265358979f * (2.0f + gm * gm)), ma = 1.0f + gm * gm,
+              mb = 2.0f * gm;
+  const float4* __restrict__ geo = P.sgeo + ih * ndirs;
   float3 acc = make_float3(0.f, 0.f, 0.f);
   for (int j = threadIdx.x; j < ndirs; j += blockDim.x) {
-    Dir d = dirs[j];
-    float ol = (float)d.x * lx + (float)d.y * ly;
-    Lerp Le = make_lerp((float)elevation_to_index(P.g, NE, r0, r0 * d.x, true), NE);
-    Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
+    const float4 g = geo[j];
+    const float ol = fmaf(g.x, lx, g.y * ly);
+    const Lerp Le = make_lerp(g.z, NE);
+    const Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
     float3 v = lookup4(dS, NE, NL, NA, Lh, Le, Ls, La);
     if (mie) {
-      float3 m = lookup4(mie, NE, NL, NA, Lh, Le, Ls, La);
-      float ph = phase_f(P.g_mie, ol);
-      v.x += m.x * ph; v.y += m.y * ph; v.z += m.z * ph;
+      const float3 m = lookup4(mie, NE, NL, NA, Lh, Le, Ls, La);
+      const float y = __builtin_amdgcn_rsqf(fmaf(-mb, ol, ma));  // phase (atmosphere.clj:56-61), x^-3/2 = rsq(x)^3
+      const float ph = mk * fmaf(ol, ol, 1.0f) * (y * y * y);
+      v.x = fmaf(m.x, ph, v.x);
+      v.y = fmaf(m.y, ph, v.y);
+      v.z = fmaf(m.z, ph, v.z);
     }
-    float w = (float)(d.x * d.w);  // (omega . normal) * quadrature weight, normal = (1 0 0)
-    acc.x += v.x * w; acc.y += v.y * w; acc.z += v.z * w;
+    acc.x = fmaf(v.x, g.w, acc.x);
+    acc.y = fmaf(v.y, g.w, acc.y);
+    acc.z = fmaf(v.z, g.w, acc.z);
   }
   // wavefront reduction, then across the workgroup's wavefronts
   for (int off = 32; off > 0; off >>= 1) {
@@ -1428,11 +1448,18 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   return hipGetLastError();
 }
 
-hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
-                                   float4* out, WorkCounters* wc, hipStream_t stream) {
+hipError_t launch_surface_geometry(const DevParams& P, const Dir* dirs, int ndirs, hipStream_t stream) {
+  if (!P.sgeo) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(surface_geometry_kernel, dim3(blocks_for((long long)P.nE[0] * ndirs)), dim3(kBlock), 0, stream, P,
+                     dirs, ndirs);
+  return hipGetLastError();
+}
+
+hipError_t launch_surface_radiance(const DevParams& P, int ndirs, const float4* dS, const float4* mie, float4* out,
+                                   WorkCounters* wc, hipStream_t stream) {
+  if (!P.sgeo) return hipErrorInvalidValue;
   unsigned entries = (unsigned)(P.nE[0] * P.nE[1]);
-  hipLaunchKernelGGL(surface_radiance_kernel, dim3(entries), dim3(kBlock), 0, stream, P, dirs, ndirs, dS, mie, out,
-                     wc);
+  hipLaunchKernelGGL(surface_radiance_kernel, dim3(entries), dim3(kBlock), 0, stream, P, ndirs, dS, mie, out, wc);
   return hipGetLastError();
 }
 

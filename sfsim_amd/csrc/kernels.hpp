@@ -29,8 +29,11 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
 // Reads P.ray_geo (launch_ray_geometry).
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream);
-hipError_t launch_surface_radiance(const DevParams& P, const Dir* dirs, int ndirs, const float4* dS, const float4* mie,
-                                   float4* out, WorkCounters* wc, hipStream_t stream);
+// Half-sphere direction geometry of every surface-radiance height into P.sgeo (once per build, before
+// launch_surface_radiance).
+hipError_t launch_surface_geometry(const DevParams& P, const Dir* dirs, int ndirs, hipStream_t stream);
+hipError_t launch_surface_radiance(const DevParams& P, int ndirs, const float4* dS, const float4* mie, float4* out,
+                                   WorkCounters* wc, hipStream_t stream);
 // Re-sample a (+ b when non-null) at forward(backward(i)) of the space (2, 3 or 4).  Output either as
 // float4 grid-order table (out4) or as packed RGB floats (out_rgb): grid order, or for space 4 with
 // tiled != 0 the 4D->2D tiling of the .scatter files.
