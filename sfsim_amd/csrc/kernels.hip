@@ -930,7 +930,7 @@ __device__ __forceinline__ float4 ld4b(const float4* base, unsigned byte_offset)
 // Sum over the quadrature directions of one entry (sun direction l, sun-elevation lerp Ls; row_bytes:
 // byte offset of row Ls.i0 in a slab).  Heading lerp: lower column min(floor, NA - 2), so the upper
 // column is always the next one; the dE lookup likewise in both dimensions (nested as lookup2).
-template <bool kRow>
+template <bool kRow, bool kMie>
 __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const DirLds* s_dir, int ndirs,
                                                      const float4* __restrict__ sh, const float4* __restrict__ shm,
                                                      const float4* __restrict__ dE, int NA, int NLA, Lerp Ls,
@@ -950,7 +950,7 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
       const float3 u = lerp3(ld4b(row, off + d_row), ld4b(row, off + d_row + 16u), sa);
       v = make_float3(fmaf(Ls.s, u.x - v.x, v.x), fmaf(Ls.s, u.y - v.y, v.y), fmaf(Ls.s, u.z - v.z, v.z));
     }
-    if (shm) {
+    if (kMie) {
       const float4* rowm = shm + j * NLA;
       float3 m = lerp3(ld4b(rowm, off), ld4b(rowm, off + 16u), sa);
       if (!kRow) {
@@ -983,6 +983,7 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
   return acc;
 }
 
+template <bool kMie>
 __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
                                                                const DirGeom* __restrict__ geom, int ndirs,
                                                                const float4* __restrict__ slab,
@@ -1029,7 +1030,7 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   pc.mie_b = 2.0f * gm;
   __syncthreads();
   const float4* sh = slab + (long long)ih * ndirs * NLA;
-  const float4* shm = slab_mie ? slab_mie + (long long)ih * ndirs * NLA : nullptr;
+  const float4* shm = kMie ? slab_mie + (long long)ih * ndirs * NLA : nullptr;
   const long long base = (long long)rayid * NLA;
   unsigned long long l4 = 0, l2 = 0;
   for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
@@ -1043,9 +1044,9 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
     // a grid entry's own sun-elevation coordinate is an integer, so its weight is exactly zero and only
     // one row of each slab is read; the general lookup stays for safety
     const float3 acc = Ls.s == 0.0f
-                           ? point_scatter_dirs<true>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz)
-                           : point_scatter_dirs<false>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz);
-    l4 += (unsigned long long)ndirs * (shm ? 2 : 1);
+                           ? point_scatter_dirs<true, kMie>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz)
+                           : point_scatter_dirs<false, kMie>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz);
+    l4 += (unsigned long long)ndirs * (kMie ? 2 : 1);
     if (wc)
       for (int j = 0; j < ndirs; ++j) l2 += s_dir[j].omega.w != 0.f;
     out[base + k] = make_float4(acc.x, acc.y, acc.z, 0.f);
@@ -1409,8 +1410,12 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
   if (e != hipSuccess) return e;
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   size_t shmem = sizeof(DirLds) * (size_t)ndirs;
-  hipLaunchKernelGGL(point_scatter_kernel, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, (const DirGeom*)geom, ndirs,
-                     slab, mie ? slab_mie : nullptr, dE, out, wc);
+  if (mie)
+    hipLaunchKernelGGL(point_scatter_kernel<true>, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, (const DirGeom*)geom,
+                       ndirs, slab, slab_mie, dE, out, wc);
+  else
+    hipLaunchKernelGGL(point_scatter_kernel<false>, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, (const DirGeom*)geom,
+                       ndirs, slab, nullptr, dE, out, wc);
   return hipGetLastError();
 }
 
