@@ -1,14 +1,15 @@
 // HIP kernels (gfx950) of the atmosphere lookup-table precompute.
 //
 // Mapping: one table entry per lane.  The 4D kernels run one workgroup per view ray
-// (height index, elevation index) of the scatter grid: the ray geometry, the per-sample altitudes
-// and the view-ray transmittance T(x, p_i) depend only on the ray, so the workgroup computes them
-// once into LDS and every lane (sun-elevation, heading entry) of that ray reuses them.
-// Quadratures follow the reference exactly (midpoint nodes of ray.clj:19-30, latitude rings of
-// sphere.clj:62-105, O(n^2) transmittance of atmosphere.clj:192-200); only loop-invariant work is
-// shared.  Geometry (intersections, the sun direction at each sample) is FP64; the extinction
-// sums along rays are FP32 with compensated accumulation, switching to FP64 for segments whose
-// lowest point is interior and below DevParams::graze_height.
+// (height index, elevation index) of the scatter grid, in XCD-aware order: the ray geometry, the
+// per-sample altitudes and the view-ray transmittance T(x, p_i) depend only on the ray, so they are
+// computed once per build (ray_geometry_kernel) and every lane (sun-elevation, heading entry) of that
+// ray reuses them.  Quadratures follow the reference (midpoint nodes of ray.clj:19-30, latitude rings
+// of sphere.clj:62-105, O(n^2) transmittance of atmosphere.clj:192-200); only loop-invariant work is
+// shared, and single scattering reads each sample's sun optical depth from a table of the same node
+// sums (sun_depth_kernel).  Geometry (intersections, the sun direction at each sample) is FP64; the
+// extinction sums along rays are FP32 with compensated accumulation, switching to FP64 for segments
+// whose lowest point is interior and below DevParams::graze_height.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -562,7 +563,7 @@ __global__ void __launch_bounds__(kBlock) ray_geometry_kernel(const DevParams P,
 
 // Single scatter with the sun segments' optical depths read from the sun_depth table: one block per
 // view ray, one thread per (sun elevation, heading) entry, each adding its lit samples in sample order.
-// The view segments T(x, p_i) and the sample geometry are computed once per ray into LDS.
+// The ray's samples (t, h, T(x, p_i), sun-table terms) come from the per-build geometry through LDS.
 constexpr int kSSTBlock = 256;
 
 __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
@@ -1419,10 +1420,9 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
   return hipGetLastError();
 }
 
-// Samples per folded-slab chunk of the ray-scatter kernel: about 2048 slab entries (32 KB of LDS) by
-// default; SFSIM_ATMO_RAY_CHUNK (1..8) overrides it.
-// Samples per chunk of the ray-scatter kernel: its folded slabs take Q x NL x NA float4 of LDS, kept
-// within 32 KiB (plus 8 KiB of per-sample offsets and weights).
+// Samples per folded-slab chunk of the ray-scatter kernel: Q x NL x NA float4 of LDS, about 2048 slab
+// entries (32 KiB, plus 8 KiB of per-sample offsets and weights); SFSIM_ATMO_RAY_CHUNK (1..8, within the
+// same LDS bound) overrides it.
 int ray_scatter_chunk(const DevParams& P) {
   int nla = P.nS[2] * P.nS[3];
   int q = 2048 / nla;
