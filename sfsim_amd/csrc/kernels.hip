@@ -698,25 +698,98 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
   return acc;
 }
 
+// Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1).
+// With one cell per thread, consecutive samples in the same (height, elevation) cell reuse the four
+// corner values already loaded (the offsets are workgroup-uniform, so is the branch).
+__device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const int4* s_off, const float4* s_wt,
+                                                 float4* slab, int i0, int qn, int NLA, int t, int nt) {
+  if (NLA <= nt) {
+    const int kk = t;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, d = a;
+    int last = -1;
+    for (int q = 0; q < qn; ++q) {
+      const int4 o = s_off[i0 + q];
+      const float4 w = s_wt[i0 + q];
+      const int key = __builtin_amdgcn_readfirstlane(o.x);
+      if (key != last && kk < NLA) {
+        a = dJ[key + kk];
+        b = dJ[__builtin_amdgcn_readfirstlane(o.y) + kk];
+        c = dJ[__builtin_amdgcn_readfirstlane(o.z) + kk];
+        d = dJ[__builtin_amdgcn_readfirstlane(o.w) + kk];
+      }
+      last = key;
+      if (kk < NLA)
+        slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
+                                         fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
+                                         fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+    }
+  } else {
+    for (int q = 0; q < qn; ++q) {
+      const int4 o = s_off[i0 + q];
+      const float4 w = s_wt[i0 + q];
+      const float4* __restrict__ d0 = dJ + __builtin_amdgcn_readfirstlane(o.x);
+      const float4* __restrict__ d1 = dJ + __builtin_amdgcn_readfirstlane(o.y);
+      const float4* __restrict__ d2 = dJ + __builtin_amdgcn_readfirstlane(o.z);
+      const float4* __restrict__ d3 = dJ + __builtin_amdgcn_readfirstlane(o.w);
+      for (int kk = t; kk < NLA; kk += nt) {
+        const float4 a = d0[kk], b = d1[kk], c = d2[kk], d = d3[kk];
+        slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
+                                         fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
+                                         fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+      }
+    }
+  }
+}
+
+// Warp-specialised: waves 0-3 fold, waves 4-7 look up.  The folded slabs are double-buffered: while the
+// lookup waves read chunk c from one half, the fold waves fill the other half with chunk c + 1, and one
+// barrier per chunk hands both halves over (chunk c + 1 overwrites the half that chunk c - 1's lookups
+// read, finished before the previous barrier).
+constexpr int kRSBlock = 512;
+constexpr int kRSHalf = kRSBlock / 2;
+
 template <int EPT>
-__global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
-                                                             const float4* __restrict__ geo_T,
-                                                             const float4* __restrict__ geo_rs,
-                                                             float4* __restrict__ out, int Q, WorkCounters* wc) {
+__global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
+                                                               const float4* __restrict__ geo_T,
+                                                               const float4* __restrict__ geo_rs,
+                                                               float4* __restrict__ out, int Q, WorkCounters* wc) {
   __shared__ int4 s_off[kMaxSteps];    // slab offsets (in units of NL x NA) of every sample
   __shared__ float4 s_wt[kMaxSteps];   // their bilinear weights (h0 e0, h0 e1, h1 e0, h1 e1)
-  extern __shared__ float4 s_slab[];   // Q x (NL x NA)
+  extern __shared__ float4 s_slab[];   // 2 x Q x (NL x NA): chunk c in half c % 2
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps;
   const int rayid = xcd_ray(blockIdx.x, NH * NE);
   const int ih = rayid / NE, ie = rayid - ih * NE;
-  const Ray ray = make_ray(P, ih, ie);
-  const double dt = ray.D / n;
+  const bool folder = threadIdx.x < kRSHalf;  // whole wavefronts
+  const int t = folder ? (int)threadIdx.x : (int)threadIdx.x - kRSHalf;
   // this ray's samples in the per-build geometry: T(x, p_i) rgb and (height coordinate, elevation
   // coordinate, alpha_i, beta_i)
   const float4* __restrict__ gT = geo_T + (long long)rayid * n;
   const float4* __restrict__ gRS = geo_rs + (long long)rayid * n;
+  // element offsets of the four (height, elevation) slabs of every sample and their weights
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float4 g = gRS[i];
+    const Lerp Lh = make_lerp(g.x, NH), Le = make_lerp(g.y, NE);
+    s_off[i] = make_int4((Lh.i0 * NE + Le.i0) * NLA, (Lh.i0 * NE + Le.i1) * NLA, (Lh.i1 * NE + Le.i0) * NLA,
+                         (Lh.i1 * NE + Le.i1) * NLA);
+    s_wt[i] = make_float4((1.0f - Le.s) * (1.0f - Lh.s), Le.s * (1.0f - Lh.s), (1.0f - Le.s) * Lh.s, Le.s * Lh.s);
+  }
+  __syncthreads();
+  const int chunks = (n + Q - 1) / Q;
+  if (folder) {
+    ray_scatter_fold(dJ, s_off, s_wt, s_slab, 0, min(Q, n), NLA, t, kRSHalf);
+    __syncthreads();
+    for (int c = 0; c < chunks; ++c) {
+      if (c + 1 < chunks)
+        ray_scatter_fold(dJ, s_off, s_wt, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
+                         t, kRSHalf);
+      __syncthreads();
+    }
+    return;
+  }
+  const Ray ray = make_ray(P, ih, ie);
+  const double dt = ray.D / n;
   const SunCoord sc = {(float)((NL - 1) * (double)P.sun_inv_den), (float)(-0.6 * 1.4426950408889634), (float)(NL - 1)};
   const float row_bytes = (float)(NA * (int)sizeof(float4));
   // per-entry constants: heading coordinate and the terms of dot(p_i, l)
@@ -725,7 +798,7 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
   float3 acc[EPT];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
-    int k = threadIdx.x + q * blockDim.x;
+    int k = t + q * kRSHalf;
     acc[q] = make_float3(0.f, 0.f, 0.f);
     vl[q] = 0.f;
     xl[q] = 0.f;
@@ -740,61 +813,13 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
       La[q] = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
     }
   }
-  // element offsets of the four (height, elevation) slabs of every sample and their weights, shared
-  // by the workgroup
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const float4 g = gRS[i];
-    const Lerp Lh = make_lerp(g.x, NH), Le = make_lerp(g.y, NE);
-    s_off[i] = make_int4((Lh.i0 * NE + Le.i0) * NLA, (Lh.i0 * NE + Le.i1) * NLA, (Lh.i1 * NE + Le.i0) * NLA,
-                         (Lh.i1 * NE + Le.i1) * NLA);
-    s_wt[i] = make_float4((1.0f - Le.s) * (1.0f - Lh.s), Le.s * (1.0f - Lh.s), (1.0f - Le.s) * Lh.s, Le.s * Lh.s);
-  }
-  __syncthreads();
-  for (int i0 = 0; i0 < n; i0 += Q) {
-    const int qn = min(Q, n - i0);
-    float4* slab = s_slab;
-    if (NLA <= (int)blockDim.x) {
-      // one cell per thread: consecutive samples in the same (height, elevation) cell reuse the four
-      // corner values already loaded (the offsets are workgroup-uniform, so is the branch)
-      const int kk = threadIdx.x;
-      float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, d = a;
-      int last = -1;
-      for (int q = 0; q < qn; ++q) {
-        const int4 o = s_off[i0 + q];
-        const float4 w = s_wt[i0 + q];
-        const int key = __builtin_amdgcn_readfirstlane(o.x);
-        if (key != last && kk < NLA) {
-          a = dJ[key + kk];
-          b = dJ[__builtin_amdgcn_readfirstlane(o.y) + kk];
-          c = dJ[__builtin_amdgcn_readfirstlane(o.z) + kk];
-          d = dJ[__builtin_amdgcn_readfirstlane(o.w) + kk];
-        }
-        last = key;
-        if (kk < NLA)
-          slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
-                                           fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
-                                           fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
-      }
-    } else {
-      for (int q = 0; q < qn; ++q) {
-        const int4 o = s_off[i0 + q];
-        const float4 w = s_wt[i0 + q];
-        const float4* __restrict__ d0 = dJ + __builtin_amdgcn_readfirstlane(o.x);
-        const float4* __restrict__ d1 = dJ + __builtin_amdgcn_readfirstlane(o.y);
-        const float4* __restrict__ d2 = dJ + __builtin_amdgcn_readfirstlane(o.z);
-        const float4* __restrict__ d3 = dJ + __builtin_amdgcn_readfirstlane(o.w);
-        for (int kk = threadIdx.x; kk < NLA; kk += blockDim.x) {
-          const float4 a = d0[kk], b = d1[kk], c = d2[kk], d = d3[kk];
-          slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
-                                           fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
-                                           fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
-        }
-      }
-    }
-    __syncthreads();
+  __syncthreads();  // chunk 0 folded
+  for (int c = 0; c < chunks; ++c) {
+    const int i0 = c * Q, qn = min(Q, n - i0);
+    const float4* slab = s_slab + (c & 1) * Q * NLA;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      if (threadIdx.x + e * blockDim.x < NLA) {
+      if (t + e * kRSHalf < NLA) {
         const float col_bytes = (float)(La[e].i0 * (int)sizeof(float4));
         // a heading coordinate with zero weight (every entry but the clamped ones) reads one column;
         // the choice is made per wavefront so that a wavefront never runs both loops
@@ -813,7 +838,7 @@ __global__ void __launch_bounds__(kBlock) ray_scatter_kernel(const DevParams P, 
   unsigned long long lookups = 0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int k = threadIdx.x + e * blockDim.x;
+    const int k = t + e * kRSHalf;
     if (k < NLA) {
       out[base + k] = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
       lookups += n;
@@ -1420,17 +1445,17 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
   return hipGetLastError();
 }
 
-// Samples per folded-slab chunk of the ray-scatter kernel: Q x NL x NA float4 of LDS, about 2048 slab
-// entries (32 KiB, plus 8 KiB of per-sample offsets and weights); SFSIM_ATMO_RAY_CHUNK (1..8, within the
-// same LDS bound) overrides it.
+// Samples per folded-slab chunk of the ray-scatter kernel: its two slab halves take 2 x Q x NL x NA float4
+// of LDS, about 2 x 1024 slab entries (32 KiB, plus 8 KiB of per-sample offsets and weights), so that four
+// 512-thread workgroups fit a CU; SFSIM_ATMO_RAY_CHUNK (1..8, within the same LDS bound) overrides it.
 int ray_scatter_chunk(const DevParams& P) {
   int nla = P.nS[2] * P.nS[3];
-  int q = 2048 / nla;
+  int q = 1024 / nla;
   if (const char* env = std::getenv("SFSIM_ATMO_RAY_CHUNK")) {
     int v = std::atoi(env);
     if (v >= 1) q = v;
   }
-  const int qmax = std::min(kMaxRayChunk, 2048 / nla);
+  const int qmax = std::min(kMaxRayChunk, 1024 / nla);
   return q < 1 ? 1 : (q > qmax ? std::max(qmax, 1) : q);
 }
 
@@ -1440,14 +1465,14 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   int nla = P.nS[2] * P.nS[3];
   int Q = ray_scatter_chunk(P);
-  size_t shmem = sizeof(float4) * (size_t)Q * nla;
+  size_t shmem = 2 * sizeof(float4) * (size_t)Q * nla;
   const RayGeoView V = ray_geo_view(P);
-  if (nla <= kBlock)
-    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
-  else if (nla <= 2 * kBlock)
-    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
-  else if (nla <= 4 * kBlock)
-    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
+  if (nla <= kRSHalf)
+    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
+  else if (nla <= 2 * kRSHalf)
+    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
+  else if (nla <= 4 * kRSHalf)
+    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
