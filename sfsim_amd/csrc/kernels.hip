@@ -1009,7 +1009,8 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
   return acc;
 }
 
-template <bool kMie>
+// kELds: the previous order's dE table is staged in LDS after the direction table (when it fits 16 KiB).
+template <bool kMie, bool kELds>
 __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
                                                                const DirGeom* __restrict__ geom, int ndirs,
                                                                const float4* __restrict__ slab,
@@ -1017,9 +1018,12 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
                                                                const float4* __restrict__ dE, float4* __restrict__ out,
                                                                WorkCounters* wc) {
   extern __shared__ DirLds s_dir[];
+  float4* s_dE = (float4*)(s_dir + ndirs);
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int rayid = xcd_ray(blockIdx.x, NH * NE);
+  if (kELds)
+    for (int i = threadIdx.x; i < P.nE[0] * P.nE[1]; i += blockDim.x) s_dE[i] = dE[i];
   const int ih = rayid / NE, ie = rayid - ih * NE;
   const double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
   double se, ce;
@@ -1055,6 +1059,7 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   pc.mie_a = 1.0f + gm * gm;
   pc.mie_b = 2.0f * gm;
   __syncthreads();
+  const float4* eTab = kELds ? s_dE : dE;
   const float4* sh = slab + (long long)ih * ndirs * NLA;
   const float4* shm = kMie ? slab_mie + (long long)ih * ndirs * NLA : nullptr;
   const long long base = (long long)rayid * NLA;
@@ -1070,8 +1075,8 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
     // a grid entry's own sun-elevation coordinate is an integer, so its weight is exactly zero and only
     // one row of each slab is read; the general lookup stays for safety
     const float3 acc = Ls.s == 0.0f
-                           ? point_scatter_dirs<true, kMie>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz)
-                           : point_scatter_dirs<false, kMie>(pc, s_dir, ndirs, sh, shm, dE, NA, NLA, Ls, row_bytes, lx, ly, lz);
+                           ? point_scatter_dirs<true, kMie>(pc, s_dir, ndirs, sh, shm, eTab, NA, NLA, Ls, row_bytes, lx, ly, lz)
+                           : point_scatter_dirs<false, kMie>(pc, s_dir, ndirs, sh, shm, eTab, NA, NLA, Ls, row_bytes, lx, ly, lz);
     l4 += (unsigned long long)ndirs * (kMie ? 2 : 1);
     if (wc)
       for (int j = 0; j < ndirs; ++j) l2 += s_dir[j].omega.w != 0.f;
@@ -1435,13 +1440,22 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
-  size_t shmem = sizeof(DirLds) * (size_t)ndirs;
-  if (mie)
-    hipLaunchKernelGGL(point_scatter_kernel<true>, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, (const DirGeom*)geom,
-                       ndirs, slab, slab_mie, dE, out, wc);
+  const size_t e_bytes = sizeof(float4) * (size_t)P.nE[0] * P.nE[1];
+  const bool e_lds = e_bytes <= 16384;
+  const size_t shmem = sizeof(DirLds) * (size_t)ndirs + (e_lds ? e_bytes : 0);
+  const DirGeom* g = (const DirGeom*)geom;
+  if (mie && e_lds)
+    hipLaunchKernelGGL((point_scatter_kernel<true, true>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs, slab,
+                       slab_mie, dE, out, wc);
+  else if (mie)
+    hipLaunchKernelGGL((point_scatter_kernel<true, false>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
+                       slab, slab_mie, dE, out, wc);
+  else if (e_lds)
+    hipLaunchKernelGGL((point_scatter_kernel<false, true>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
+                       slab, nullptr, dE, out, wc);
   else
-    hipLaunchKernelGGL(point_scatter_kernel<false>, dim3(rays), dim3(kBlock), shmem, stream, P, dirs, (const DirGeom*)geom,
-                       ndirs, slab, nullptr, dE, out, wc);
+    hipLaunchKernelGGL((point_scatter_kernel<false, false>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
+                       slab, nullptr, dE, out, wc);
   return hipGetLastError();
 }
 
