@@ -2,6 +2,7 @@
 // generate-atmosphere-luts driver (atmosphere_lut.clj:43-105) and the per-table entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -74,6 +75,8 @@ struct sfsim_atmo_ctx {
   DeviceBuffer ray_geo;  // per-(view ray, sample) geometry of single and ray scatter
   DeviceBuffer rmap;     // per-entry 4D re-sampling lookups of the pipelines
   DeviceBuffer sgeo;     // half-sphere direction geometry per surface-radiance height
+  DeviceBuffer ray_order;  // ray-scatter visiting order (ascending impact parameter)
+  double ray_order_key[4] = {-1.0, -1.0, -1.0, -1.0};  // (R, H, NH, NE) it was computed for
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
   Surrogate sg;
@@ -302,7 +305,37 @@ DevParams dev_params(const sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   P.ray_geo = (char*)ctx->ray_geo.ptr;
   P.rmap = (char*)ctx->rmap.ptr;  // filled by launch_resample_map at the start of a pipeline
   P.sgeo = (float4*)ctx->sgeo.ptr;  // filled by launch_surface_geometry before the surface-radiance stages
+  P.ray_order = (const int*)ctx->ray_order.ptr;  // set by ensure_ray_order
   return P;
+}
+
+// The view rays of the scatter grid in ascending impact parameter r0 sqrt(1 - mu0^2) (ties in grid
+// order), uploaded when the planet or the grid changes.
+int ensure_ray_order(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
+  const double key[4] = {p->planet.radius, p->planet.height, (double)p->scatter_shape[0],
+                         (double)p->scatter_shape[1]};
+  if (ctx->ray_order.ptr && std::memcmp(key, ctx->ray_order_key, sizeof(key)) == 0) return 0;
+  const DevParams P = make_dev_params(p);
+  const int NH = p->scatter_shape[0], NE = p->scatter_shape[1];
+  std::vector<double> b((size_t)NH * NE);
+  for (int ih = 0; ih < NH; ++ih) {
+    const double r0 = index_to_radius(P.g, NH, ih);
+    for (int ie = 0; ie < NE; ++ie) {
+      double se, ce;
+      bool above;
+      index_to_elevation(P.g, NE, r0, ie, se, ce, above);
+      b[(size_t)ih * NE + ie] = r0 * std::fabs(ce);
+    }
+  }
+  std::vector<int> order(b.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return b[u] < b[v]; });
+  TRY(ensure(ctx, ctx->ray_order, order.size() * sizeof(int)));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->ray_order.ptr, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(ctx->ray_order_key, key, sizeof(key));
+  return 0;
 }
 
 // The surface-radiance direction geometry buffer (after ensure_dirs for the half sphere).
@@ -318,6 +351,7 @@ int ensure_sun_depth(sfsim_atmo_ctx* ctx) {
 // The sun table and the per-(view ray, sample) geometry buffer of the single/ray-scatter kernels.
 int ensure_scatter_buffers(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(ensure_sun_depth(ctx));
+  TRY(ensure_ray_order(ctx, p));
   TRY(ensure(ctx, ctx->rmap, resample_map_bytes(make_dev_params(p))));
   return ensure(ctx, ctx->ray_geo, ray_geometry_bytes(make_dev_params(p)));
 }
@@ -900,7 +934,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo, &ctx->ray_order};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);

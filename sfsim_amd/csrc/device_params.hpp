@@ -51,6 +51,9 @@ struct DevParams {
   // per (surface-radiance height, half-sphere direction): omega.x, omega.y, elevation coordinate of
   // omega at the height's grid point, cosine-weighted quadrature weight (nE[0] x directions float4)
   float4* sgeo;
+  // order in which the ray-scatter kernel visits the view rays (NH x NE ray indices, ascending impact
+  // parameter |x| sqrt(1 - mu^2)); null: grid order
+  const int* ray_order;
 };
 
 __host__ __device__ inline size_t resample_map_bytes(const DevParams& P) {

@@ -759,7 +759,11 @@ __global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps;
-  const int rayid = xcd_ray(blockIdx.x, NH * NE);
+  // XCD-aware, in ascending impact parameter: every sample of a straight ray has the same
+  // |p| sqrt(1 - mu^2), so the (height, elevation) cells that a ray folds lie along one curve, and an XCD
+  // given a contiguous range of impact parameters reads a band of the dJ table
+  const int ord = xcd_ray(blockIdx.x, NH * NE);
+  const int rayid = P.ray_order ? P.ray_order[ord] : ord;
   const int ih = rayid / NE, ie = rayid - ih * NE;
   const bool folder = threadIdx.x < kRSHalf;  // whole wavefronts
   const int t = folder ? (int)threadIdx.x : (int)threadIdx.x - kRSHalf;
