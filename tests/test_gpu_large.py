@@ -1,11 +1,17 @@
-"""The large workload: S 64x128x64x16 (8,388,608 entries), T 128x512, E 32x128, 128 ray steps, sphere
-steps 28 (256 directions, half sphere 2625), 10 scattering orders, 100 km atmosphere.
+"""Workloads beyond the full 5-order one, stage by stage:
 
-The FP64 oracle cannot run the 10-order driver at this size within a test, so every stage is checked on
-seeded entry subsets against the oracle's stage fed the GPU's own previous-stage tables (the same
-scheme as the full-workload subsets of test_gpu_parity.py), the whole build must be finite and
-physical, and two builds must agree bit for bit.  Tolerances: transmittance 1e-4, everything else 1e-3
-relative (floor: 1e-6 of the table's channel maximum)."""
+* large: S 64x128x64x16 (8,388,608 entries), T 128x512, E 32x128, 128 ray steps, sphere steps 28 (256
+  directions, half sphere 2625), 10 scattering orders, 100 km atmosphere (NL x NA = 1024: four entries per
+  thread in the 4D kernels, one sample per folded-slab chunk, dE read from global memory);
+* production (the reference's own parameters, atmosphere_lut.clj:20-62): S 32x127x32x8, T 64x255, E 16x63,
+  100 ray steps, sphere steps 15 (71 directions, half sphere 1605), 35 km (odd grid sizes, a ray-step count
+  that is not a multiple of the chunk or of 64).
+
+The FP64 oracle cannot run the multi-order driver at these sizes within a test, so every stage is checked
+on seeded entry subsets against the oracle's stage fed the GPU's own previous-stage tables (the same
+scheme as the full-workload subsets of test_gpu_parity.py), the whole build must be finite and physical,
+and two builds must agree bit for bit.  Tolerances: transmittance 1e-4, everything else 1e-3 relative
+(floor: 1e-6 of the table's channel maximum)."""
 import numpy as np
 import pytest
 
@@ -27,9 +33,9 @@ def gpu(atm):
     ctx.close()
 
 
-@pytest.fixture(scope="module")
-def large(atm, gpu):
-    P = atm.workload("large-10-order")
+@pytest.fixture(scope="module", params=["large-10-order", "production"])
+def large(atm, gpu, request):
+    P = atm.workload(request.param)
     T = gpu.transmittance_table(P)
     E0 = gpu.surface_radiance_base_table(P)
     R1, M1 = gpu.single_scatter_tables(P)
@@ -85,8 +91,9 @@ def test_large_surface_radiance(oracle, large):
     assert_close("dE1 large", dE1.reshape(-1, 3)[idx], ref, TOL_SCATTER)
 
 
-def test_large_build_is_finite_physical_and_deterministic(atm, gpu):
-    P = atm.workload("large-10-order")
+@pytest.mark.parametrize("name", ["large-10-order", "production"])
+def test_large_build_is_finite_physical_and_deterministic(atm, gpu, name):
+    P = atm.workload(name)
     gpu.build(P)
     first = [gpu.fetch(P, k) for k in range(4)]
     gpu.build(P)
