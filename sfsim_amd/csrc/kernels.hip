@@ -601,11 +601,15 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
     const double vl = ray.se * l0 + ray.ce * l1;
     const double xl = ray.r0 * l0;
     float3 accR = make_float3(0.f, 0.f, 0.f), accM = make_float3(0.f, 0.f, 0.f);
+    // branch-free: a sample with the sun below the horizon (atmosphere.clj:154-160) reads the table at
+    // clamped, in-range coordinates and adds exactly zero, so consecutive samples' reads overlap
+#pragma unroll 2
     for (int i = 0; i < n; ++i) {
       const double2 th = s_th[i];
       const double h = th.y, pl = xl + th.x * vl;  // dot(p_i, l)
-      if (!is_above_horizon(P.g, h, pl)) continue;  // atmosphere.clj:154-160
-      ++lit;
+      const bool sun = is_above_horizon(P.g, h, pl);
+      lit += sun;
+      const float on = sun ? 1.0f : 0.0f;
       const float4 gm = s_g[i];
       const float2 rr = s_r[i];
       const float plf = (float)pl;
@@ -621,7 +625,7 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
       const float sm = fexp2(fmaf(rr.x, hi_m - lo_m, lo_m)), sr = fexp2(fmaf(rr.x, hi_r - lo_r, lo_r));
       const float3 Ts = transmittance_of(P, sm, sr, 1.0);
       const float4 w = s_w[i];
-      const float3 t3 = make_float3(w.x * Ts.x, w.y * Ts.y, w.z * Ts.z);
+      const float3 t3 = make_float3(w.x * Ts.x * on, w.y * Ts.y * on, w.z * Ts.z * on);
       accR.x += t3.x * gm.w; accR.y += t3.y * gm.w; accR.z += t3.z * gm.w;
       accM.x += t3.x * w.w; accM.y += t3.y * w.w; accM.z += t3.z * w.w;
     }
