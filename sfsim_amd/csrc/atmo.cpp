@@ -76,6 +76,7 @@ struct sfsim_atmo_ctx {
   DeviceBuffer rmap;     // per-entry 4D re-sampling lookups of the pipelines
   DeviceBuffer sgeo;     // half-sphere direction geometry per surface-radiance height
   DeviceBuffer ray_order;  // ray-scatter visiting order (ascending impact parameter)
+  DeviceBuffer Ssum;       // S + dS at the grid points, written by ray scatter for the S accumulation
   double ray_order_key[4] = {-1.0, -1.0, -1.0, -1.0};  // (R, H, NH, NE) it was computed for
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
@@ -498,6 +499,7 @@ int prepare_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(ensure(ctx, ctx->R1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->M1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->dJ, nS * sizeof(float4)));
+  if (p->iterations > 0) TRY(ensure(ctx, ctx->Ssum, nS * sizeof(float4)));
   const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
   for (int i = 0; i < 4; ++i) {
     TRY(ensure(ctx, ctx->out[i], outc[i] * sizeof(float)));
@@ -552,7 +554,8 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
     LAUNCH(timer, kSurfaceRadiance,
            launch_surface_radiance(P, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
     const int dS_new = dS_cur < 0 ? 0 : 1 - dS_cur;
-    LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[dS_new]), wc, ctx->stream));
+    LAUNCH(timer, kRayScatter,
+           launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[dS_new]), wc, ctx->stream, S_cur, f4(ctx->Ssum)));
     dE_cur = dE_new;
     dS_cur = dS_new;
     if (capture) {
@@ -572,7 +575,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
     E_cur = f4(ctx->Eacc[e_next]);
     e_next = 1 - e_next;
     LAUNCH(timer, kAccumulate,
-           launch_resample(P, 4, S_cur, f4(ctx->dS[dS_cur]), f4(ctx->Sacc[s_next]), nullptr, 0, ctx->stream));
+           launch_resample(P, 4, f4(ctx->Ssum), nullptr, f4(ctx->Sacc[s_next]), nullptr, 0, ctx->stream));
     S_cur = f4(ctx->Sacc[s_next]);
     s_next = 1 - s_next;
   }
@@ -934,7 +937,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo, &ctx->ray_order};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);

@@ -756,7 +756,9 @@ template <int EPT>
 __global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
                                                                const float4* __restrict__ geo_T,
                                                                const float4* __restrict__ geo_rs,
-                                                               float4* __restrict__ out, int Q, WorkCounters* wc) {
+                                                               float4* __restrict__ out,
+                                                               const float4* __restrict__ acc_in,
+                                                               float4* __restrict__ acc_out, int Q, WorkCounters* wc) {
   __shared__ int4 s_off[kMaxSteps];    // slab offsets (in units of NL x NA) of every sample
   __shared__ float4 s_wt[kMaxSteps];   // their bilinear weights (h0 e0, h0 e1, h1 e0, h1 e1)
   extern __shared__ float4 s_slab[];   // 2 x Q x (NL x NA): chunk c in half c % 2
@@ -848,7 +850,12 @@ __global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P
   for (int e = 0; e < EPT; ++e) {
     const int k = t + e * kRSHalf;
     if (k < NLA) {
-      out[base + k] = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
+      const float4 d = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
+      out[base + k] = d;
+      if (acc_out) {  // the running sum S + dS of atmosphere_lut.clj:95-97 at this grid point
+        const float4 u = acc_in[base + k];
+        acc_out[base + k] = make_float4(u.x + d.x, u.y + d.y, u.z + d.z, 0.f);
+      }
       lookups += n;
     }
   }
@@ -1482,7 +1489,8 @@ int ray_scatter_chunk(const DevParams& P) {
 }
 
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
-                              hipStream_t stream) {
+                              hipStream_t stream, const float4* acc_in, float4* acc_out) {
+  if ((acc_in == nullptr) != (acc_out == nullptr)) return hipErrorInvalidValue;
   if (!P.ray_geo) return hipErrorInvalidValue;
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   int nla = P.nS[2] * P.nS[3];
@@ -1490,11 +1498,14 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   size_t shmem = 2 * sizeof(float4) * (size_t)Q * nla;
   const RayGeoView V = ray_geo_view(P);
   if (nla <= kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc_in,
+                       acc_out, Q, wc);
   else if (nla <= 2 * kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc_in,
+                       acc_out, Q, wc);
   else if (nla <= 4 * kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc_in,
+                       acc_out, Q, wc);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -26,9 +26,10 @@ size_t direction_geometry_bytes(const DevParams& P, int ndirs);
 hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
                                 const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
                                 WorkCounters* wc, hipStream_t stream);
-// Reads P.ray_geo (launch_ray_geometry).
+// Reads P.ray_geo (launch_ray_geometry).  With acc_in / acc_out (both or neither) it also writes
+// acc_out = acc_in + out at every grid point (the sum that the next S accumulation re-samples).
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
-                              hipStream_t stream);
+                              hipStream_t stream, const float4* acc_in = nullptr, float4* acc_out = nullptr);
 // Half-sphere direction geometry of every surface-radiance height into P.sgeo (once per build, before
 // launch_surface_radiance).
 hipError_t launch_surface_geometry(const DevParams& P, const Dir* dirs, int ndirs, hipStream_t stream);
