@@ -1114,11 +1114,16 @@ __global__ void __launch_bounds__(kBlock) surface_geometry_kernel(const DevParam
                             (float)(d.x * d.w));  // (omega . normal) * quadrature weight, normal = (1 0 0)
 }
 
+// kFold: the entry's height and sun-elevation coordinates are the same for every direction, so the
+// workgroup first folds those two dimensions of dS (and of the Mie strength) into an (elevation,
+// heading) slab in LDS, NE x NA float4, and each direction interpolates the other two in it.
+template <bool kFold>
 __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P, int ndirs,
                                                                   const float4* __restrict__ dS,
                                                                   const float4* __restrict__ mie,
                                                                   float4* __restrict__ out, WorkCounters* wc) {
   __shared__ float3 s_red[kBlock / 64];
+  extern __shared__ float4 s_fold[];  // kFold: [NE x NA] dS, then [NE x NA] Mie strength
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int ih = blockIdx.x / P.nE[1], il = blockIdx.x - ih * P.nE[1];
   const double r0 = index_to_radius(P.g, P.nE[0], ih);
@@ -1131,15 +1136,34 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParam
   const float mk = 3.0f * (1.0f - gm * gm) / (8.0f * 3.14159265358979f * (2.0f + gm * gm)), ma = 1.0f + gm * gm,
               mb = 2.0f * gm;
   const float4* __restrict__ geo = P.sgeo + ih * ndirs;
+  const int NEA = NE * NA;
+  float4* s_m = s_fold + NEA;
+  if (kFold) {
+    const long long h0 = (long long)Lh.i0 * NE * NL * NA, h1 = (long long)Lh.i1 * NE * NL * NA;
+    const int s0 = Ls.i0 * NA, s1 = Ls.i1 * NA;
+    for (int c = threadIdx.x; c < NEA; c += blockDim.x) {
+      const int e = c / NA, a = c - e * NA;
+      const long long eo = (long long)e * NL * NA + a;
+      const float3 u = mix3(mix3(ld3(dS, h0 + eo + s0), ld3(dS, h0 + eo + s1), Ls.s),
+                            mix3(ld3(dS, h1 + eo + s0), ld3(dS, h1 + eo + s1), Ls.s), Lh.s);
+      s_fold[c] = make_float4(u.x, u.y, u.z, 0.f);
+      if (mie) {
+        const float3 m = mix3(mix3(ld3(mie, h0 + eo + s0), ld3(mie, h0 + eo + s1), Ls.s),
+                              mix3(ld3(mie, h1 + eo + s0), ld3(mie, h1 + eo + s1), Ls.s), Lh.s);
+        s_m[c] = make_float4(m.x, m.y, m.z, 0.f);
+      }
+    }
+    __syncthreads();
+  }
   float3 acc = make_float3(0.f, 0.f, 0.f);
   for (int j = threadIdx.x; j < ndirs; j += blockDim.x) {
     const float4 g = geo[j];
     const float ol = fmaf(g.x, lx, g.y * ly);
     const Lerp Le = make_lerp(g.z, NE);
     const Lerp La = make_lerp((NA - 1) * ((1.0f + ol) * 0.5f), NA);
-    float3 v = lookup4(dS, NE, NL, NA, Lh, Le, Ls, La);
+    float3 v = kFold ? lookup2(s_fold, NA, Le, La) : lookup4(dS, NE, NL, NA, Lh, Le, Ls, La);
     if (mie) {
-      const float3 m = lookup4(mie, NE, NL, NA, Lh, Le, Ls, La);
+      const float3 m = kFold ? lookup2(s_m, NA, Le, La) : lookup4(mie, NE, NL, NA, Lh, Le, Ls, La);
       const float y = __builtin_amdgcn_rsqf(fmaf(-mb, ol, ma));  // phase (atmosphere.clj:56-61), x^-3/2 = rsq(x)^3
       const float ph = mk * fmaf(ol, ol, 1.0f) * (y * y * y);
       v.x = fmaf(m.x, ph, v.x);
@@ -1522,7 +1546,12 @@ hipError_t launch_surface_radiance(const DevParams& P, int ndirs, const float4* 
                                    WorkCounters* wc, hipStream_t stream) {
   if (!P.sgeo) return hipErrorInvalidValue;
   unsigned entries = (unsigned)(P.nE[0] * P.nE[1]);
-  hipLaunchKernelGGL(surface_radiance_kernel, dim3(entries), dim3(kBlock), 0, stream, P, ndirs, dS, mie, out, wc);
+  const size_t fold = sizeof(float4) * (size_t)P.nS[1] * P.nS[3] * (mie ? 2 : 1);
+  if (fold <= 65536)
+    hipLaunchKernelGGL(surface_radiance_kernel<true>, dim3(entries), dim3(kBlock), fold, stream, P, ndirs, dS, mie, out,
+                       wc);
+  else
+    hipLaunchKernelGGL(surface_radiance_kernel<false>, dim3(entries), dim3(kBlock), 0, stream, P, ndirs, dS, mie, out, wc);
   return hipGetLastError();
 }
 

@@ -153,12 +153,15 @@ def layerwise_prediction(sg, P, entries, batch):
 
 
 def assert_same_prediction(fused, layerwise):
-    """Both paths round to bf16 at the same places; only FP32 summation order differs, which can flip
-    an activation's last bf16 bit: about 1e-3 in the log outputs at worst."""
+    """Both paths round to bf16 at the same places; only FP32 summation order differs, which can flip the
+    bf16 rounding of an activation.  The log outputs (log(P + 1e-8) + 10) reach ~10, where one bf16 unit
+    (2^-8 relative) is ~4e-2: a flip moves an output by at most about that, so the bound on the largest
+    difference is 5e-2; almost everywhere the two paths agree (median ~1e-7), bounded at 2e-3 (a
+    transposed operand or a wrong tile would move every output)."""
     for k in range(5):
         d = np.abs(np.log(fused[k].astype(np.float64) + 1e-8) - np.log(layerwise[k] + 1e-8))
         print(f"P{k + 1}: max |d log| {d.max():.2e}, median {np.median(d):.2e}")
-        assert d.max() <= 2e-2 and np.median(d) <= 2e-3, k + 1
+        assert d.max() <= 5e-2 and np.median(d) <= 2e-3, k + 1
 
 
 def test_fused_prediction_matches_layerwise_full_grid(atm, net):
