@@ -27,6 +27,9 @@ constexpr int kMaxRayChunk = 8;  // samples per slab chunk of the ray-scatter ke
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// min(floor(x), hi) for a coordinate already clamped to x >= 0, as one med3 (fminf would quiet a
+// signalling NaN in hi on every call)
+__device__ __forceinline__ float lower_cell(float x, float hi) { return __builtin_amdgcn_fmed3f(floorf(x), hi, 0.0f); }
 
 // top_exit_distance (geometry.hpp) for a point inside the atmosphere (h <= H), branch-free, with the
 // FP64 square root and reciprocal from the FP32 hardware estimates refined by one Newton step each
@@ -201,6 +204,19 @@ __device__ __forceinline__ Lerp make_lerp(float c, int N) {
   L.i1 = min(L.i0 + 1, N - 1);
   L.s = i - u;
   return L;
+}
+
+// The two corners of a lerp by index parity: idx[p], w[p] for the corner whose index has parity p
+// (i0 with weight 1 - s, i1 with weight s); at the upper edge (i1 == i0, s == 0) the one node carries
+// weight 1 and the other parity is empty (idx -1, weight 0).
+__device__ __forceinline__ void corner_by_parity(const Lerp& L, int idx[2], float w[2]) {
+  const bool odd = (L.i0 & 1) != 0;
+  const int other = L.i1 != L.i0 ? L.i1 : -1;
+  const float w_other = L.i1 != L.i0 ? L.s : 0.0f, w_lower = 1.0f - L.s;
+  idx[0] = odd ? other : L.i0;
+  w[0] = odd ? w_other : w_lower;
+  idx[1] = odd ? L.i0 : other;
+  w[1] = odd ? w_lower : w_other;
 }
 
 __device__ __forceinline__ float3 mix3(float3 a, float3 b, float s) {
@@ -616,7 +632,7 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
       const float q = __builtin_amdgcn_sqrtf(fmaf(plf, plf, gm.x));
       const float D = plf > 0.f ? gm.x * __builtin_amdgcn_rcpf(plf + q) : q - plf;
       const float y = fminf(fmaxf((D - gm.y) * gm.z, 0.f), (float)(cols - 1));
-      const float col = fminf(floorf(y), (float)(cols - 2));
+      const float col = lower_cell(y, (float)(cols - 2));
       const float fy = y - col;
       const int base = __float_as_int(rr.y) + (int)col;
       const float2 v00 = tab[base], v01 = tab[base + 1], v10 = tab[base + cols], v11 = tab[base + cols + 1];
@@ -683,7 +699,7 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
     const float4 w = gT[i];  // workgroup-uniform: scalar loads
     const float4 g = gRS[i];
     const float c = sun_coord(sc, make_float2(g.z, g.w), xl, vl);
-    const float row = fminf(floorf(c), sc.top - 1.0f);
+    const float row = lower_cell(c, sc.top - 1.0f);
     const float s = c - row;
     const unsigned off = (unsigned)fmaf(row, row_bytes, q_bytes);
     float3 v;
@@ -702,27 +718,52 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
   return acc;
 }
 
-// Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1).
-// With one cell per thread, consecutive samples in the same (height, elevation) cell reuse the four
-// corner values already loaded (the offsets are workgroup-uniform, so is the branch).
+// Register slots of the fold: slot ((h & 1) << 1) | (e & 1) holds dJ at node (h, e) for the folding
+// thread's entry.  The four corners of a cell have distinct parities, so every sample's corners occupy
+// the four slots, and a corner shared with the previous sample is already in its slot: a slot is
+// re-read only when its node changes (about 60 node reads per ray of 64 samples instead of four per
+// sample that changes cell).
+struct FoldSlots {
+  float4 v[4];
+  int key[4];
+};
+
+__device__ __forceinline__ void fold_slots_init(FoldSlots& f) {
+  for (int s = 0; s < 4; ++s) {
+    f.v[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    f.key[s] = -1;
+  }
+}
+
+__device__ __forceinline__ void fold_slot(const float4* __restrict__ dJ, FoldSlots& f, int s, int o, int kk,
+                                          bool active) {
+  const int key = __builtin_amdgcn_readfirstlane(o);  // -1: no node in this slot (weight zero)
+  if (key >= 0 && key != f.key[s]) {
+    if (active) f.v[s] = dJ[key + kk];
+    f.key[s] = key;
+  }
+}
+
+// Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1):
+// slab[q][kk] = sum over slots of weight x dJ[node][kk].  s_off holds per sample the element offset of
+// each slot's node (-1 when the slot is unused), s_wt their weights (zero for unused slots); the
+// offsets are workgroup-uniform, so are the branches.  With one entry per thread the slots carry over
+// from chunk to chunk.
 __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const int4* s_off, const float4* s_wt,
-                                                 float4* slab, int i0, int qn, int NLA, int t, int nt) {
+                                                 float4* slab, int i0, int qn, int NLA, int t, int nt,
+                                                 FoldSlots& f) {
   if (NLA <= nt) {
     const int kk = t;
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, d = a;
-    int last = -1;
+    const bool active = kk < NLA;
     for (int q = 0; q < qn; ++q) {
       const int4 o = s_off[i0 + q];
       const float4 w = s_wt[i0 + q];
-      const int key = __builtin_amdgcn_readfirstlane(o.x);
-      if (key != last && kk < NLA) {
-        a = dJ[key + kk];
-        b = dJ[__builtin_amdgcn_readfirstlane(o.y) + kk];
-        c = dJ[__builtin_amdgcn_readfirstlane(o.z) + kk];
-        d = dJ[__builtin_amdgcn_readfirstlane(o.w) + kk];
-      }
-      last = key;
-      if (kk < NLA)
+      fold_slot(dJ, f, 0, o.x, kk, active);
+      fold_slot(dJ, f, 1, o.y, kk, active);
+      fold_slot(dJ, f, 2, o.z, kk, active);
+      fold_slot(dJ, f, 3, o.w, kk, active);
+      const float4 a = f.v[0], b = f.v[1], c = f.v[2], d = f.v[3];
+      if (active)
         slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
                                          fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
                                          fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
@@ -731,10 +772,11 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
     for (int q = 0; q < qn; ++q) {
       const int4 o = s_off[i0 + q];
       const float4 w = s_wt[i0 + q];
-      const float4* __restrict__ d0 = dJ + __builtin_amdgcn_readfirstlane(o.x);
-      const float4* __restrict__ d1 = dJ + __builtin_amdgcn_readfirstlane(o.y);
-      const float4* __restrict__ d2 = dJ + __builtin_amdgcn_readfirstlane(o.z);
-      const float4* __restrict__ d3 = dJ + __builtin_amdgcn_readfirstlane(o.w);
+      // an unused slot (offset -1, weight 0) reads node 0, which adds zero
+      const float4* __restrict__ d0 = dJ + max(__builtin_amdgcn_readfirstlane(o.x), 0);
+      const float4* __restrict__ d1 = dJ + max(__builtin_amdgcn_readfirstlane(o.y), 0);
+      const float4* __restrict__ d2 = dJ + max(__builtin_amdgcn_readfirstlane(o.z), 0);
+      const float4* __restrict__ d3 = dJ + max(__builtin_amdgcn_readfirstlane(o.w), 0);
       for (int kk = t; kk < NLA; kk += nt) {
         const float4 a = d0[kk], b = d1[kk], c = d2[kk], d = d3[kk];
         slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
@@ -777,23 +819,33 @@ __global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P
   // coordinate, alpha_i, beta_i)
   const float4* __restrict__ gT = geo_T + (long long)rayid * n;
   const float4* __restrict__ gRS = geo_rs + (long long)rayid * n;
-  // element offsets of the four (height, elevation) slabs of every sample and their weights
+  // element offsets of the (height, elevation) slabs of every sample and their bilinear weights, by
+  // fold slot: the corner with height index parity ph and elevation index parity pe goes to slot
+  // 2 ph + pe (at the upper edge of the grid, where i1 == i0 and the weight of i1 is zero, the
+  // coinciding corners share one slot and the slots of the other parity stay unused)
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const float4 g = gRS[i];
     const Lerp Lh = make_lerp(g.x, NH), Le = make_lerp(g.y, NE);
-    s_off[i] = make_int4((Lh.i0 * NE + Le.i0) * NLA, (Lh.i0 * NE + Le.i1) * NLA, (Lh.i1 * NE + Le.i0) * NLA,
-                         (Lh.i1 * NE + Le.i1) * NLA);
-    s_wt[i] = make_float4((1.0f - Le.s) * (1.0f - Lh.s), Le.s * (1.0f - Lh.s), (1.0f - Le.s) * Lh.s, Le.s * Lh.s);
+    int hs[2], es[2];
+    float hw[2], ew[2];
+    corner_by_parity(Lh, hs, hw);
+    corner_by_parity(Le, es, ew);
+    auto off = [&](int ph, int pe) { return hs[ph] < 0 || es[pe] < 0 ? -1 : (hs[ph] * NE + es[pe]) * NLA; };
+    auto wt = [&](int ph, int pe) { return hs[ph] < 0 || es[pe] < 0 ? 0.0f : ew[pe] * hw[ph]; };
+    s_off[i] = make_int4(off(0, 0), off(0, 1), off(1, 0), off(1, 1));
+    s_wt[i] = make_float4(wt(0, 0), wt(0, 1), wt(1, 0), wt(1, 1));
   }
   __syncthreads();
   const int chunks = (n + Q - 1) / Q;
   if (folder) {
-    ray_scatter_fold(dJ, s_off, s_wt, s_slab, 0, min(Q, n), NLA, t, kRSHalf);
+    FoldSlots fs;
+    fold_slots_init(fs);
+    ray_scatter_fold(dJ, s_off, s_wt, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs);
     __syncthreads();
     for (int c = 0; c < chunks; ++c) {
       if (c + 1 < chunks)
         ray_scatter_fold(dJ, s_off, s_wt, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
-                         t, kRSHalf);
+                         t, kRSHalf, fs);
       __syncthreads();
     }
     return;
@@ -872,7 +924,7 @@ __global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P
 struct DirGeom {
   float4 omega;   // direction xyz, w: elevation coordinate of omega at x (flag: not surface)
   float4 ground;  // T(x, P) * brightness / pi (0 for sky directions), w: 1 for surface directions
-  float4 phat;    // P / |P|, w: height coordinate of P in the surface-radiance table
+  float4 phat;    // P / |P| (w unused: P is on the ground, height row 0 of the surface-radiance table)
 };
 
 __global__ void __launch_bounds__(kBlock) direction_geometry_kernel(const DevParams P, const Dir* __restrict__ dirs,
@@ -901,7 +953,7 @@ __global__ void __launch_bounds__(kBlock) direction_geometry_kernel(const DevPar
                            1.f);
     double px = r0 + d.x * D, py = d.y * D, pz = d.z * D;
     double pn = 1.0 / sqrt(px * px + py * py + pz * pz);
-    g.phat = make_float4((float)(px * pn), (float)(py * pn), (float)(pz * pn), (float)altitude_to_index(P.g, P.nE[0], hP));
+    g.phat = make_float4((float)(px * pn), (float)(py * pn), (float)(pz * pn), 0.f);
   } else {
     g.ground = make_float4(0.f, 0.f, 0.f, 0.f);
     g.phat = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -945,8 +997,8 @@ __global__ void __launch_bounds__(kBlock) point_scatter_combine_kernel(const Dev
 struct DirLds {
   float4 omega;   // direction xyz, w: 1 for surface directions
   float4 weight;  // rgb quadrature weight times in-scattering
-  float4 ground;  // T(x, P) brightness / pi, w: weight of the upper height row of P in the dE table
-  float4 phat;    // P / |P|, w: byte offset of the lower height row of P in the dE table (int bits)
+  float4 ground;  // T(x, P) brightness / pi
+  float4 phat;    // P / |P|
 };
 
 // Constants of the per-direction lookups of one point-scatter launch.
@@ -954,7 +1006,6 @@ struct PsConst {
   float hA;            // (NA - 1) / 2: heading coordinate (NA - 1) (1 + dot(omega, l)) / 2 = hA + hA dot(omega, l)
   float na_top;        // NA - 1
   SunCoord se;         // sun-elevation coordinate of dE (size nE[1]): alpha = -3 log2(e), beta = 0
-  unsigned e_row;      // bytes of one dE height row
   float mie_k, mie_a, mie_b;  // Mie phase K (1 + mu^2) (a - b mu)^(-3/2): K = 3 (1 - g^2) / (8 pi (2 + g^2))
 };
 
@@ -971,6 +1022,7 @@ __device__ __forceinline__ float4 ld4b(const float4* base, unsigned byte_offset)
 // Sum over the quadrature directions of one entry (sun direction l, sun-elevation lerp Ls; row_bytes:
 // byte offset of row Ls.i0 in a slab).  Heading lerp: lower column min(floor, NA - 2), so the upper
 // column is always the next one; the dE lookup likewise in both dimensions (nested as lookup2).
+// kRow: the entry's sun-elevation weight is zero (one slab row)
 template <bool kRow, bool kMie>
 __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const DirLds* s_dir, int ndirs,
                                                      const float4* __restrict__ sh, const float4* __restrict__ shm,
@@ -978,11 +1030,12 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
                                                      float row_bytes, float lx, float ly, float lz) {
   float3 acc = make_float3(0.f, 0.f, 0.f);
   const unsigned d_row = (unsigned)(Ls.i1 - Ls.i0) * (unsigned)NA * (unsigned)sizeof(float4);
+#pragma unroll 2
   for (int j = 0; j < ndirs; ++j) {
     const DirLds& dd = s_dir[j];
     const float ol = fmaf(dd.omega.z, lz, fmaf(dd.omega.y, ly, dd.omega.x * lx));
     const float hc = __builtin_amdgcn_fmed3f(fmaf(pc.hA, ol, pc.hA), 0.0f, pc.na_top);
-    const float col = fminf(floorf(hc), pc.na_top - 1.0f);
+    const float col = lower_cell(hc, pc.na_top - 1.0f);
     const float sa = hc - col;
     const unsigned off = (unsigned)fmaf(col, (float)sizeof(float4), row_bytes);
     const float4* row = sh + j * NLA;
@@ -1007,15 +1060,13 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
       const float spl = fmaf(dd.phat.z, lz, fmaf(dd.phat.y, ly, dd.phat.x * lx));
       const float ce = __builtin_amdgcn_fmed3f(
           fmaf(-pc.se.A, fexp2(fmaf(-3.0f * 1.4426950408889634f, spl, pc.se.c)), pc.se.A), 0.0f, pc.se.top);
-      const float ecol = fminf(floorf(ce), pc.se.top - 1.0f);
+      const float ecol = lower_cell(ce, pc.se.top - 1.0f);
       const float se = ce - ecol;
-      const unsigned eo = (unsigned)__float_as_int(dd.phat.w) + (unsigned)ecol * (unsigned)sizeof(float4);
+      const unsigned eo = (unsigned)ecol * (unsigned)sizeof(float4);  // height row 0: P is on the ground
       const float3 u = lerp3(ld4b(dE, eo), ld4b(dE, eo + 16u), se);
-      const float3 t = lerp3(ld4b(dE, eo + pc.e_row), ld4b(dE, eo + pc.e_row + 16u), se);
-      const float wh = dd.ground.w;
-      v.x = fmaf(dd.ground.x, fmaf(wh, t.x - u.x, u.x), v.x);
-      v.y = fmaf(dd.ground.y, fmaf(wh, t.y - u.y, u.y), v.y);
-      v.z = fmaf(dd.ground.z, fmaf(wh, t.z - u.z, u.z), v.z);
+      v.x = fmaf(dd.ground.x, u.x, v.x);
+      v.y = fmaf(dd.ground.y, u.y, v.y);
+      v.z = fmaf(dd.ground.z, u.z, v.z);
     }
     acc.x = fmaf(dd.weight.x, v.x, acc.x);
     acc.y = fmaf(dd.weight.y, v.y, acc.y);
@@ -1024,7 +1075,10 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
   return acc;
 }
 
-// kELds: the previous order's dE table is staged in LDS after the direction table (when it fits 16 KiB).
+// kELds: the previous order's dE at the ground (height row 0) is staged in LDS after the direction
+// table (when it fits 16 KiB).  P, the ground point of a surface direction, is the ray's intersection
+// with the planet's sphere (atmosphere.clj:80-85, 213-221), so its height is 0: its FP64 |P| - R is
+// rounding noise of ~1e-9 m, whose height coordinate (~1e-13) would weight height row 1 by as much.
 template <bool kMie, bool kELds>
 __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
                                                                const DirGeom* __restrict__ geom, int ndirs,
@@ -1038,7 +1092,7 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   const int NLA = NL * NA;
   const int rayid = xcd_ray(blockIdx.x, NH * NE);
   if (kELds)
-    for (int i = threadIdx.x; i < P.nE[0] * P.nE[1]; i += blockDim.x) s_dE[i] = dE[i];
+    for (int i = threadIdx.x; i < P.nE[1]; i += blockDim.x) s_dE[i] = dE[i];
   const int ih = rayid / NE, ie = rayid - ih * NE;
   const double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
   double se, ce;
@@ -1046,7 +1100,7 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   index_to_elevation(P.g, NE, r0, ie, se, ce, above_unused);
   const float bm = (float)exp(-h0 * P.inv_scale_mie), br = (float)exp(-h0 * P.inv_scale_ray);
   const DirGeom* gh = geom + (long long)ih * ndirs;
-  const int nE0 = P.nE[0], nE1 = P.nE[1];
+  const int nE1 = P.nE[1];
   for (int j = threadIdx.x; j < ndirs; j += blockDim.x) {
     Dir d = dirs[j];
     DirGeom g = gh[j];
@@ -1057,11 +1111,8 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
     dl.omega = make_float4(g.omega.x, g.omega.y, g.omega.z, g.ground.w);
     dl.weight = make_float4((P.sca_mie[0] * pm + P.sca_ray[0] * pr) * w, (P.sca_mie[1] * pm + P.sca_ray[1] * pr) * w,
                             (P.sca_mie[2] * pm + P.sca_ray[2] * pr) * w, 0.f);
-    // height lerp of P in the dE table (interpolate.clj:75-98), lower row min(floor, nE0 - 2)
-    const float hc = fminf(fmaxf(g.phat.w, 0.0f), (float)(nE0 - 1));
-    const float hrow = fminf(floorf(hc), (float)(nE0 - 2));
-    dl.ground = make_float4(g.ground.x, g.ground.y, g.ground.z, hc - hrow);
-    dl.phat = make_float4(g.phat.x, g.phat.y, g.phat.z, __int_as_float((int)hrow * nE1 * (int)sizeof(float4)));
+    dl.ground = make_float4(g.ground.x, g.ground.y, g.ground.z, 0.f);
+    dl.phat = make_float4(g.phat.x, g.phat.y, g.phat.z, 0.f);
     s_dir[j] = dl;
   }
   const float gm = P.g_mie;
@@ -1069,7 +1120,6 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   pc.hA = 0.5f * (float)(NA - 1);
   pc.na_top = (float)(NA - 1);
   pc.se = SunCoord{(float)((nE1 - 1) * (double)P.sun_inv_den), (float)(-0.6 * 1.4426950408889634), (float)(nE1 - 1)};
-  pc.e_row = (unsigned)nE1 * (unsigned)sizeof(float4);
   pc.mie_k = 3.0f * (1.0f - gm * gm) / (8.0f * 3.14159265358979f * (2.0f + gm * gm));
   pc.mie_a = 1.0f + gm * gm;
   pc.mie_b = 2.0f * gm;
@@ -1479,7 +1529,7 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
-  const size_t e_bytes = sizeof(float4) * (size_t)P.nE[0] * P.nE[1];
+  const size_t e_bytes = sizeof(float4) * (size_t)P.nE[1];  // dE height row 0, the ground
   const bool e_lds = e_bytes <= 16384;
   const size_t shmem = sizeof(DirLds) * (size_t)ndirs + (e_lds ? e_bytes : 0);
   const DirGeom* g = (const DirGeom*)geom;
