@@ -76,7 +76,7 @@ struct sfsim_atmo_ctx {
   DeviceBuffer rmap;     // per-entry 4D re-sampling lookups of the pipelines
   DeviceBuffer sgeo;     // half-sphere direction geometry per surface-radiance height
   DeviceBuffer ray_order;  // ray-scatter visiting order (ascending impact parameter)
-  DeviceBuffer Ssum;       // S + dS at the grid points, written by ray scatter for the S accumulation
+  DeviceBuffer Ssum[2];    // S + dS at the grid points, written by ray scatter (alternate iterations)
   double ray_order_key[4] = {-1.0, -1.0, -1.0, -1.0};  // (R, H, NH, NE) it was computed for
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
@@ -499,7 +499,8 @@ int prepare_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(ensure(ctx, ctx->R1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->M1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->dJ, nS * sizeof(float4)));
-  if (p->iterations > 0) TRY(ensure(ctx, ctx->Ssum, nS * sizeof(float4)));
+  if (p->iterations > 0)
+    for (int i = 0; i < 2; ++i) TRY(ensure(ctx, ctx->Ssum[i], nS * sizeof(float4)));
   const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
   for (int i = 0; i < 4; ++i) {
     TRY(ensure(ctx, ctx->out[i], outc[i] * sizeof(float)));
@@ -539,9 +540,14 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
     LAUNCH(timer, kDirectionGeometry, launch_surface_geometry(P, hdirs, ctx->half_n, ctx->stream));
   }
   // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
-  const float4* S_cur = f4(ctx->R1);
+  // The accumulations E <- table(E + dE), S <- table(S + dS) (atmosphere_lut.clj:95-97) ride in the
+  // ray-scatter launches: iteration k writes Ssum[k % 2] = S + dS_k, with S = R1 for the first and
+  // S = table(Ssum of the iteration before) after that, and E_k = table(dE_k + E_{k-1}) in an extra
+  // workgroup; the last S re-sampling is a launch of its own.
+  const float4* S_in = f4(ctx->R1);
+  int S_resample = 0;
   const float4* E_cur = nullptr;
-  int dE_cur = 0, dS_cur = -1, s_next = 0, e_next = 0;
+  int dE_cur = 0, dS_cur = -1, e_next = 0;
   for (int it = 0; it < p->iterations; ++it) {
     const bool first = it == 0;
     const float4* dS_tab = first ? f4(ctx->R1) : f4(ctx->dS[dS_cur]);
@@ -554,10 +560,20 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
     LAUNCH(timer, kSurfaceRadiance,
            launch_surface_radiance(P, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
     const int dS_new = dS_cur < 0 ? 0 : 1 - dS_cur;
-    LAUNCH(timer, kRayScatter,
-           launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[dS_new]), wc, ctx->stream, S_cur, f4(ctx->Ssum)));
+    RsAccum acc;
+    acc.S_in = S_in;
+    acc.S_out = f4(ctx->Ssum[it & 1]);
+    acc.S_resample = S_resample;
+    acc.dE = f4(ctx->dE[dE_new]);
+    acc.E_in = E_cur;
+    acc.E_out = f4(ctx->Eacc[e_next]);
+    LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), f4(ctx->dS[dS_new]), wc, ctx->stream, acc));
     dE_cur = dE_new;
     dS_cur = dS_new;
+    E_cur = f4(ctx->Eacc[e_next]);
+    e_next = 1 - e_next;
+    S_in = f4(ctx->Ssum[it & 1]);
+    S_resample = 1;
     if (capture) {
       const float4* d = f4(ctx->dS[dS_cur]);
       float4* sum = f4(ctx->sumS);
@@ -569,15 +585,11 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
       if (order <= kMlpOrders)
         HIP_TRY(ctx, hipMemcpyAsync(capture[order - 1], sum, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
     }
-    // E <- table(E + dE), S <- table(S + dS) (atmosphere_lut.clj:95-97)
-    LAUNCH(timer, kAccumulate,
-           launch_resample(P, 3, f4(ctx->dE[dE_cur]), E_cur, f4(ctx->Eacc[e_next]), nullptr, 0, ctx->stream));
-    E_cur = f4(ctx->Eacc[e_next]);
-    e_next = 1 - e_next;
-    LAUNCH(timer, kAccumulate,
-           launch_resample(P, 4, f4(ctx->Ssum), nullptr, f4(ctx->Sacc[s_next]), nullptr, 0, ctx->stream));
-    S_cur = f4(ctx->Sacc[s_next]);
-    s_next = 1 - s_next;
+  }
+  const float4* S_cur = S_in;  // R1 without iterations
+  if (S_resample) {
+    LAUNCH(timer, kAccumulate, launch_resample(P, 4, S_in, nullptr, f4(ctx->Sacc[0]), nullptr, 0, ctx->stream));
+    S_cur = f4(ctx->Sacc[0]);
   }
   // final make-lookup-table calls and file layouts (atmosphere_lut.clj:98-105)
   LAUNCH(timer, kFinal, launch_resample(P, 2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, ctx->stream));
@@ -937,7 +949,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum[0], &ctx->Ssum[1]};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);

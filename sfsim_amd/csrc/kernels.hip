@@ -787,6 +787,11 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
   }
 }
 
+__device__ float3 resample4_value(const DevParams& P, const float4* __restrict__ a, const float4* __restrict__ b,
+                                 long long idx);
+__device__ float3 resample2_value(const DevParams& P, int space, const float4* __restrict__ a,
+                                 const float4* __restrict__ b, long long idx);
+
 // Warp-specialised: waves 0-3 fold, waves 4-7 look up.  The folded slabs are double-buffered: while the
 // lookup waves read chunk c from one half, the fold waves fill the other half with chunk c + 1, and one
 // barrier per chunk hands both halves over (chunk c + 1 overwrites the half that chunk c - 1's lookups
@@ -794,19 +799,28 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
 constexpr int kRSBlock = 512;
 constexpr int kRSHalf = kRSBlock / 2;
 
+// Eight waves per SIMD (four workgroups per CU, as the LDS allows): at most 64 VGPRs, the re-sampling
+// of the accumulations spilling a few registers outside the loops.
 template <int EPT>
-__global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
+__global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
                                                                const float4* __restrict__ geo_T,
                                                                const float4* __restrict__ geo_rs,
-                                                               float4* __restrict__ out,
-                                                               const float4* __restrict__ acc_in,
-                                                               float4* __restrict__ acc_out, int Q, WorkCounters* wc) {
+                                                               float4* __restrict__ out, const RsAccum accum, int Q,
+                                                               WorkCounters* wc) {
   __shared__ int4 s_off[kMaxSteps];    // slab offsets (in units of NL x NA) of every sample
   __shared__ float4 s_wt[kMaxSteps];   // their bilinear weights (h0 e0, h0 e1, h1 e0, h1 e1)
   extern __shared__ float4 s_slab[];   // 2 x Q x (NL x NA): chunk c in half c % 2
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps;
+  if ((int)blockIdx.x >= NH * NE) {  // the extra workgroup: E <- tab(dE + E) (atmosphere_lut.clj:95-96)
+    const long long nE = (long long)P.nE[0] * P.nE[1];
+    for (long long i = threadIdx.x; i < nE; i += blockDim.x) {
+      const float3 v = resample2_value(P, 3, accum.dE, accum.E_in, i);
+      accum.E_out[i] = make_float4(v.x, v.y, v.z, 0.f);
+    }
+    return;
+  }
   // XCD-aware, in ascending impact parameter: every sample of a straight ray has the same
   // |p| sqrt(1 - mu^2), so the (height, elevation) cells that a ray folds lie along one curve, and an XCD
   // given a contiguous range of impact parameters reads a band of the dJ table
@@ -904,9 +918,10 @@ __global__ void __launch_bounds__(kRSBlock) ray_scatter_kernel(const DevParams P
     if (k < NLA) {
       const float4 d = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
       out[base + k] = d;
-      if (acc_out) {  // the running sum S + dS of atmosphere_lut.clj:95-97 at this grid point
-        const float4 u = acc_in[base + k];
-        acc_out[base + k] = make_float4(u.x + d.x, u.y + d.y, u.z + d.z, 0.f);
+      if (accum.S_out) {  // the running sum S + dS of atmosphere_lut.clj:95-97 at this grid point
+        const float3 u =
+            accum.S_resample ? resample4_value(P, accum.S_in, nullptr, base + k) : ld3(accum.S_in, base + k);
+        accum.S_out[base + k] = make_float4(u.x + d.x, u.y + d.y, u.z + d.z, 0.f);
       }
       lookups += n;
     }
@@ -1287,76 +1302,95 @@ __global__ void __launch_bounds__(kBlock) resample_map_kernel(const DevParams P)
       make_ushort4((unsigned short)Lh.i0, (unsigned short)Le.i0, (unsigned short)Ls.i0, (unsigned short)La.i0);
 }
 
+// Value at 4D entry idx of tab(a + b) (b nullable): the multilinear lookup at forward(backward(idx)),
+// from the per-build map when there is one.
+__device__ float3 resample4_value(const DevParams& P, const float4* __restrict__ a, const float4* __restrict__ b,
+                                 long long idx) {
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const long long n = (long long)NH * NE * NL * NA;
+  Lerp Lh, Le, Ls, La;
+  if (P.rmap) {
+    const float4 w = ((const float4*)P.rmap)[idx];
+    const ushort4 i = ((const ushort4*)(P.rmap + n * sizeof(float4)))[idx];
+    Lh = lerp_of(i.x, w.x, NH);
+    Le = lerp_of(i.y, w.y, NE);
+    Ls = lerp_of(i.z, w.z, NL);
+    La = lerp_of(i.w, w.w, NA);
+  } else {
+    const int ia = (int)(idx % NA);
+    long long q = idx / NA;
+    const int is = (int)(q % NL);
+    q /= NL;
+    resample4_lerps(P, (int)(q / NE), (int)(q % NE), is, ia, Lh, Le, Ls, La);
+  }
+  float3 v;
+  if (Lh.s == 0.0f && Le.s == 0.0f && Ls.s == 0.0f && La.s == 0.0f) {
+    // the entry maps onto a grid point (every weight zero): the lookup is that point's value, bit for bit
+    const long long c = (((long long)Lh.i0 * NE + Le.i0) * NL + Ls.i0) * NA + La.i0;
+    v = ld3(a, c);
+    if (b) {
+      const float3 w = ld3(b, c);
+      v.x += w.x; v.y += w.y; v.z += w.z;
+    }
+  } else {
+    v = lookup4(a, NE, NL, NA, Lh, Le, Ls, La);
+    if (b) {
+      float3 w = lookup4(b, NE, NL, NA, Lh, Le, Ls, La);
+      v.x += w.x; v.y += w.y; v.z += w.z;
+    }
+  }
+  return v;
+}
+
+// Value at 2D entry idx of tab(a + b) (b nullable) for the transmittance (space 2) or
+// surface-radiance (space 3) table.
+__device__ float3 resample2_value(const DevParams& P, int space, const float4* __restrict__ a,
+                                 const float4* __restrict__ b, long long idx) {
+  const int N0 = space == 2 ? P.nT[0] : P.nE[0], N1 = space == 2 ? P.nT[1] : P.nE[1];
+  int i0 = (int)(idx / N1), i1 = (int)(idx % N1);
+  double r0 = index_to_radius(P.g, N0, i0);
+  double c1;
+  if (space == 2) {
+    double se, ce;
+    bool above;
+    index_to_elevation(P.g, N1, r0, i1, se, ce, above);
+    c1 = elevation_to_index(P.g, N1, r0, r0 * se, above);
+  } else {
+    double s = index_to_sin_sun_elevation(N1, i1);
+    c1 = sun_elevation_to_index(N1, (r0 * s) / r0);
+  }
+  Lerp L0 = make_lerp((float)height_to_index(P.g, N0, r0), N0);
+  Lerp L1 = make_lerp((float)c1, N1);
+  float3 v = lookup2_skip(a, N1, L0, L1);
+  if (b) {
+    float3 w = lookup2_skip(b, N1, L0, L1);
+    v.x += w.x; v.y += w.y; v.z += w.z;
+  }
+  return v;
+}
+
 __global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int space, const float4* __restrict__ a,
                                                           const float4* __restrict__ b, float4* __restrict__ out4,
                                                           float* __restrict__ out_rgb, int tiled) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   float3 v;
-  long long dst;
+  long long dst = idx;
   if (space == 4) {
     const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
-    const long long n = (long long)NH * NE * NL * NA;
-    if (idx >= n) return;
-    Lerp Lh, Le, Ls, La;
-    int ih = 0, ie = 0, is = 0, ia = 0;
-    if (!P.rmap || tiled) {
-      ia = (int)(idx % NA);
+    if (idx >= (long long)NH * NE * NL * NA) return;
+    v = resample4_value(P, a, b, idx);
+    if (tiled) {  // image.clj:299-312: row h * NL + s, column e * NA + a
+      const int ia = (int)(idx % NA);
       long long q = idx / NA;
-      is = (int)(q % NL);
+      const int is = (int)(q % NL);
       q /= NL;
-      ie = (int)(q % NE);
-      ih = (int)(q / NE);
+      const int ie = (int)(q % NE), ih = (int)(q / NE);
+      dst = (long long)(ih * NL + is) * (NE * NA) + (long long)ie * NA + ia;
     }
-    if (P.rmap) {
-      const float4 w = ((const float4*)P.rmap)[idx];
-      const ushort4 i = ((const ushort4*)(P.rmap + n * sizeof(float4)))[idx];
-      Lh = lerp_of(i.x, w.x, NH);
-      Le = lerp_of(i.y, w.y, NE);
-      Ls = lerp_of(i.z, w.z, NL);
-      La = lerp_of(i.w, w.w, NA);
-    } else {
-      resample4_lerps(P, ih, ie, is, ia, Lh, Le, Ls, La);
-    }
-    if (Lh.s == 0.0f && Le.s == 0.0f && Ls.s == 0.0f && La.s == 0.0f) {
-      // the entry maps onto a grid point (every weight zero): the lookup is that point's value, bit for bit
-      const long long c = (((long long)Lh.i0 * NE + Le.i0) * NL + Ls.i0) * NA + La.i0;
-      v = ld3(a, c);
-      if (b) {
-        const float3 w = ld3(b, c);
-        v.x += w.x; v.y += w.y; v.z += w.z;
-      }
-    } else {
-      v = lookup4(a, NE, NL, NA, Lh, Le, Ls, La);
-      if (b) {
-        float3 w = lookup4(b, NE, NL, NA, Lh, Le, Ls, La);
-        v.x += w.x; v.y += w.y; v.z += w.z;
-      }
-    }
-    // image.clj:299-312: row h * NL + s, column e * NA + a
-    dst = tiled ? ((long long)(ih * NL + is) * (NE * NA) + (long long)ie * NA + ia) : idx;
   } else {
     const int N0 = space == 2 ? P.nT[0] : P.nE[0], N1 = space == 2 ? P.nT[1] : P.nE[1];
     if (idx >= (long long)N0 * N1) return;
-    int i0 = (int)(idx / N1), i1 = (int)(idx % N1);
-    double r0 = index_to_radius(P.g, N0, i0);
-    double c1;
-    if (space == 2) {
-      double se, ce;
-      bool above;
-      index_to_elevation(P.g, N1, r0, i1, se, ce, above);
-      c1 = elevation_to_index(P.g, N1, r0, r0 * se, above);
-    } else {
-      double s = index_to_sin_sun_elevation(N1, i1);
-      c1 = sun_elevation_to_index(N1, (r0 * s) / r0);
-    }
-    Lerp L0 = make_lerp((float)height_to_index(P.g, N0, r0), N0);
-    Lerp L1 = make_lerp((float)c1, N1);
-    v = lookup2_skip(a, N1, L0, L1);
-    if (b) {
-      float3 w = lookup2_skip(b, N1, L0, L1);
-      v.x += w.x; v.y += w.y; v.z += w.z;
-    }
-    dst = idx;
+    v = resample2_value(P, space, a, b, idx);
   }
   if (out4) out4[idx] = make_float4(v.x, v.y, v.z, 0.f);
   if (out_rgb) {
@@ -1563,23 +1597,23 @@ int ray_scatter_chunk(const DevParams& P) {
 }
 
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
-                              hipStream_t stream, const float4* acc_in, float4* acc_out) {
-  if ((acc_in == nullptr) != (acc_out == nullptr)) return hipErrorInvalidValue;
+                              hipStream_t stream, const RsAccum& acc) {
+  if ((acc.S_in == nullptr) != (acc.S_out == nullptr) || (acc.E_out && !acc.dE)) return hipErrorInvalidValue;
   if (!P.ray_geo) return hipErrorInvalidValue;
-  unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
+  const unsigned rays = (unsigned)(P.nS[0] * P.nS[1]) + (acc.E_out ? 1u : 0u);
   int nla = P.nS[2] * P.nS[3];
   int Q = ray_scatter_chunk(P);
   size_t shmem = 2 * sizeof(float4) * (size_t)Q * nla;
   const RayGeoView V = ray_geo_view(P);
   if (nla <= kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc_in,
-                       acc_out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc, Q,
+                       wc);
   else if (nla <= 2 * kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc_in,
-                       acc_out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc, Q,
+                       wc);
   else if (nla <= 4 * kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc_in,
-                       acc_out, Q, wc);
+    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc, Q,
+                       wc);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -26,10 +26,21 @@ size_t direction_geometry_bytes(const DevParams& P, int ndirs);
 hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
                                 const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
                                 WorkCounters* wc, hipStream_t stream);
-// Reads P.ray_geo (launch_ray_geometry).  With acc_in / acc_out (both or neither) it also writes
-// acc_out = acc_in + out at every grid point (the sum that the next S accumulation re-samples).
+// The accumulations of atmosphere_lut.clj:95-97 that ride in a ray-scatter launch.  S_out (nullable,
+// with S_in): the running sum S + dS at every grid point, S = S_in, or tab(S_in) when S_resample (S_in
+// then holds the previous iteration's S + dS).  E_out (nullable): E_out = tab(dE + E_in) over the
+// surface-radiance table (E_in nullable), by one extra workgroup.
+struct RsAccum {
+  const float4* S_in = nullptr;
+  float4* S_out = nullptr;
+  int S_resample = 0;
+  const float4* dE = nullptr;
+  const float4* E_in = nullptr;
+  float4* E_out = nullptr;
+};
+// Reads P.ray_geo (launch_ray_geometry).
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
-                              hipStream_t stream, const float4* acc_in = nullptr, float4* acc_out = nullptr);
+                              hipStream_t stream, const RsAccum& acc = RsAccum());
 // Half-sphere direction geometry of every surface-radiance height into P.sgeo (once per build, before
 // launch_surface_radiance).
 hipError_t launch_surface_geometry(const DevParams& P, const Dir* dirs, int ndirs, hipStream_t stream);
