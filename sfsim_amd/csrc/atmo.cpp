@@ -609,20 +609,23 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
         HIP_TRY(ctx, hipMemcpyAsync(capture[order - 1], sum, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
     }
   }
-  const float4* S_cur = S_in;  // R1 without iterations
-  if (S_resample) {
-    LAUNCH(timer, kAccumulate, launch_resample(P, 4, S_in, nullptr, f4(ctx->Sacc[0]), nullptr, 0, ctx->stream));
-    S_cur = f4(ctx->Sacc[0]);
+  // the last S re-sampling and the final make-lookup-table calls and file layouts
+  // (atmosphere_lut.clj:97-105), the independent ones in one launch: T, E and M1 beside S <- table(S + dS)
+  ResampleJob jobs[kMaxResampleJobs];
+  int nj = 0;
+  jobs[nj++] = ResampleJob{2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0};
+  jobs[nj++] = ResampleJob{4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1};
+  if (E_cur) jobs[nj++] = ResampleJob{3, E_cur, nullptr, nullptr, (float*)ctx->out[1].ptr, 0};
+  if (S_resample) {  // S_in: the last iteration's S + dS
+    jobs[nj++] = ResampleJob{4, S_in, nullptr, f4(ctx->Sacc[0]), nullptr, 0};
+    LAUNCH(timer, kFinal, launch_resamples(P, jobs, nj, ctx->stream));
+    LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->Sacc[0]), nullptr, nullptr, (float*)ctx->out[2].ptr, 1,
+                                          ctx->stream));
+  } else {  // S = R1
+    jobs[nj++] = ResampleJob{4, S_in, nullptr, nullptr, (float*)ctx->out[2].ptr, 1};
+    LAUNCH(timer, kFinal, launch_resamples(P, jobs, nj, ctx->stream));
   }
-  // final make-lookup-table calls and file layouts (atmosphere_lut.clj:98-105)
-  LAUNCH(timer, kFinal, launch_resample(P, 2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, ctx->stream));
-  if (E_cur) {
-    LAUNCH(timer, kFinal, launch_resample(P, 3, E_cur, nullptr, nullptr, (float*)ctx->out[1].ptr, 0, ctx->stream));
-  } else {
-    LAUNCH(timer, kFinal, launch_fill_zero((float*)ctx->out[1].ptr, (long long)ctx->out_count[1], ctx->stream));
-  }
-  LAUNCH(timer, kFinal, launch_resample(P, 4, S_cur, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
-  LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
+  if (!E_cur) LAUNCH(timer, kFinal, launch_fill_zero((float*)ctx->out[1].ptr, (long long)ctx->out_count[1], ctx->stream));
   return 0;
 }
 

@@ -1369,10 +1369,9 @@ __device__ float3 resample2_value(const DevParams& P, int space, const float4* _
   return v;
 }
 
-__global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int space, const float4* __restrict__ a,
-                                                          const float4* __restrict__ b, float4* __restrict__ out4,
-                                                          float* __restrict__ out_rgb, int tiled) {
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void resample_entry(const DevParams& P, int space, const float4* __restrict__ a,
+                                               const float4* __restrict__ b, float4* __restrict__ out4,
+                                               float* __restrict__ out_rgb, int tiled, long long idx) {
   float3 v;
   long long dst = idx;
   if (space == 4) {
@@ -1398,6 +1397,27 @@ __global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int
     out_rgb[dst * 3 + 1] = v.y;
     out_rgb[dst * 3 + 2] = v.z;
   }
+}
+
+__global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int space, const float4* __restrict__ a,
+                                                          const float4* __restrict__ b, float4* __restrict__ out4,
+                                                          float* __restrict__ out_rgb, int tiled) {
+  resample_entry(P, space, a, b, out4, out_rgb, tiled, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Several independent re-samplings in one launch: job k takes workgroups first[k] .. first[k + 1] - 1.
+struct ResampleJobs {
+  ResampleJob job[kMaxResampleJobs];
+  unsigned first[kMaxResampleJobs + 1];
+  int count;
+};
+
+__global__ void __launch_bounds__(kBlock) resample_jobs_kernel(const DevParams P, const ResampleJobs J) {
+  int k = 0;
+  while (k + 1 < J.count && blockIdx.x >= J.first[k + 1]) ++k;
+  const ResampleJob& j = J.job[k];
+  resample_entry(P, j.space, j.a, j.b, j.out4, j.out_rgb, j.tiled,
+                 (long long)(blockIdx.x - J.first[k]) * blockDim.x + threadIdx.x);
 }
 
 __global__ void pack_rgb_kernel(const float4* __restrict__ in, float* __restrict__ out, long long count) {
@@ -1646,11 +1666,31 @@ hipError_t launch_resample_map(const DevParams& P, hipStream_t stream) {
   return hipGetLastError();
 }
 
+static long long resample_count(const DevParams& P, int space) {
+  return space == 4   ? (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3]
+         : space == 2 ? (long long)P.nT[0] * P.nT[1]
+                      : (long long)P.nE[0] * P.nE[1];
+}
+
+hipError_t launch_resamples(const DevParams& P, const ResampleJob* jobs, int count, hipStream_t stream) {
+  if (count < 1 || count > kMaxResampleJobs) return hipErrorInvalidValue;
+  ResampleJobs J;
+  J.count = count;
+  J.first[0] = 0;
+  for (int k = 0; k < kMaxResampleJobs; ++k) {
+    J.job[k] = jobs[k < count ? k : 0];
+    if (k < count) {
+      if (jobs[k].space < 2 || jobs[k].space > 4 || !jobs[k].a) return hipErrorInvalidValue;
+      J.first[k + 1] = J.first[k] + blocks_for(resample_count(P, jobs[k].space));
+    }
+  }
+  hipLaunchKernelGGL(resample_jobs_kernel, dim3(J.first[count]), dim3(kBlock), 0, stream, P, J);
+  return hipGetLastError();
+}
+
 hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
                            float* out_rgb, int tiled, hipStream_t stream) {
-  long long n = space == 4   ? (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3]
-                : space == 2 ? (long long)P.nT[0] * P.nT[1]
-                             : (long long)P.nE[0] * P.nE[1];
+  const long long n = resample_count(P, space);
   hipLaunchKernelGGL(resample_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, space, a, b, out4, out_rgb,
                      tiled);
   return hipGetLastError();

@@ -54,6 +54,18 @@ hipError_t launch_surface_radiance(const DevParams& P, int ndirs, const float4* 
 hipError_t launch_resample_map(const DevParams& P, hipStream_t stream);
 hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
                            float* out_rgb, int tiled, hipStream_t stream);
+// Up to kMaxResampleJobs independent launch_resample jobs in one launch (no job may write what another
+// reads).
+struct ResampleJob {
+  int space;
+  const float4* a;
+  const float4* b;
+  float4* out4;
+  float* out_rgb;
+  int tiled;
+};
+constexpr int kMaxResampleJobs = 4;
+hipError_t launch_resamples(const DevParams& P, const ResampleJob* jobs, int count, hipStream_t stream);
 // c = a + b (float4 tables)
 hipError_t launch_add4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream);
 // c = a - b (RGB, w = 0)
