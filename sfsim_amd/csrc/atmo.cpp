@@ -575,13 +575,17 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
     const bool first = it == 0;
     const float4* dS_tab = first ? f4(ctx->R1) : f4(ctx->dS[dS_cur]);
     const float4* mie = first ? f4(ctx->M1) : nullptr;
-    // the partials of atmosphere_lut.clj:88-89 bind the previous dS and dE
+    // the partials of atmosphere_lut.clj:88-89 bind the previous dS and dE; the surface radiance of
+    // dS rides in the point-scatter launch when its LDS slab fits
+    const int dE_new = 1 - dE_cur;
+    const bool sr_along = sr_fits_point_scatter(P, mie != nullptr);
+    const SrJob sr{0, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new])};
     LAUNCH(timer, kPointScatter,
            launch_point_scatter(P, sdirs, ctx->sphere_n, ctx->geom.ptr, dS_tab, mie, f4(ctx->dE[dE_cur]), f4(ctx->slab),
-                                f4(ctx->slab_mie), f4(ctx->dJ), wc, ctx->stream));
-    const int dE_new = 1 - dE_cur;
-    LAUNCH(timer, kSurfaceRadiance,
-           launch_surface_radiance(P, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
+                                f4(ctx->slab_mie), f4(ctx->dJ), wc, ctx->stream, sr_along ? &sr : nullptr));
+    if (!sr_along)
+      LAUNCH(timer, kSurfaceRadiance,
+             launch_surface_radiance(P, ctx->half_n, dS_tab, mie, f4(ctx->dE[dE_new]), wc, ctx->stream));
     const int dS_new = dS_cur < 0 ? 0 : 1 - dS_cur;
     RsAccum acc;
     acc.S_in = S_in;
@@ -793,12 +797,15 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   // the last, dE[1] = dE_1 + ... + dE_5
   for (int a = 0; a < m; ++a) {
     float4* Pk = f4(ctx->pred[j + a]);
+    const bool sr_along = sr_fits_point_scatter(P, true);
+    const SrJob sr{0, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1])};
     LAUNCH(timer, kPointScatter,
            launch_point_scatter(P, sdirs, ctx->sphere_n, ctx->geom.ptr, A, f4(ctx->M1), Esum, f4(ctx->slab),
-                                f4(ctx->slab_mie), f4(ctx->dJ), nullptr, ctx->stream));
+                                f4(ctx->slab_mie), f4(ctx->dJ), nullptr, ctx->stream, sr_along ? &sr : nullptr));
     LAUNCH(timer, kRayScatter, launch_ray_scatter(P, f4(ctx->dJ), Pk, nullptr, ctx->stream));
-    LAUNCH(timer, kSurfaceRadiance,
-           launch_surface_radiance(P, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
+    if (!sr_along)
+      LAUNCH(timer, kSurfaceRadiance,
+             launch_surface_radiance(P, ctx->half_n, A, f4(ctx->M1), f4(ctx->dE[1]), nullptr, ctx->stream));
     if (a + 1 < m) {
       LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->R1), Pk, A, n4, ctx->stream));
       LAUNCH(timer, kAccumulate, launch_add4(f4(ctx->dE[0]), f4(ctx->dE[1]), Esum, e4, ctx->stream));

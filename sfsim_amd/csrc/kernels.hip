@@ -1094,17 +1094,30 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
 // table (when it fits 16 KiB).  P, the ground point of a surface direction, is the ray's intersection
 // with the planet's sphere (atmosphere.clj:80-85, 213-221), so its height is 0: its FP64 |P| - R is
 // rounding noise of ~1e-9 m, whose height coordinate (~1e-13) would weight height row 1 by as much.
+template <bool kFold>
+__device__ __forceinline__ void surface_radiance_entry(const DevParams& P, int ndirs, const float4* __restrict__ dS,
+                                                       const float4* __restrict__ mie, float4* __restrict__ out,
+                                                       WorkCounters* wc, int entry, float4* s_fold, float3* s_red);
+
+// The iteration's surface radiance (it reads the same dS and Mie strength) rides in the launch: the
+// sr.count workgroups after the rays' compute its entries with the LDS-folded kernel body.
 template <bool kMie, bool kELds>
 __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
                                                                const DirGeom* __restrict__ geom, int ndirs,
                                                                const float4* __restrict__ slab,
                                                                const float4* __restrict__ slab_mie,
                                                                const float4* __restrict__ dE, float4* __restrict__ out,
-                                                               WorkCounters* wc) {
+                                                               const SrJob sr, WorkCounters* wc) {
   extern __shared__ DirLds s_dir[];
   float4* s_dE = (float4*)(s_dir + ndirs);
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
+  if ((int)blockIdx.x >= NH * NE) {
+    __shared__ float3 s_red[kBlock / 64];
+    surface_radiance_entry<true>(P, sr.ndirs, sr.dS, sr.mie, sr.out, wc, (int)blockIdx.x - NH * NE, (float4*)s_dir,
+                                 s_red);
+    return;
+  }
   const int rayid = xcd_ray(blockIdx.x, NH * NE);
   if (kELds)
     for (int i = threadIdx.x; i < P.nE[1]; i += blockDim.x) s_dE[i] = dE[i];
@@ -1182,15 +1195,14 @@ __global__ void __launch_bounds__(kBlock) surface_geometry_kernel(const DevParam
 // kFold: the entry's height and sun-elevation coordinates are the same for every direction, so the
 // workgroup first folds those two dimensions of dS (and of the Mie strength) into an (elevation,
 // heading) slab in LDS, NE x NA float4, and each direction interpolates the other two in it.
+// One workgroup computes entry `entry` of the table; s_fold: kFold's LDS slab, s_red: one float3 per
+// wavefront.
 template <bool kFold>
-__global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P, int ndirs,
-                                                                  const float4* __restrict__ dS,
-                                                                  const float4* __restrict__ mie,
-                                                                  float4* __restrict__ out, WorkCounters* wc) {
-  __shared__ float3 s_red[kBlock / 64];
-  extern __shared__ float4 s_fold[];  // kFold: [NE x NA] dS, then [NE x NA] Mie strength
+__device__ __forceinline__ void surface_radiance_entry(const DevParams& P, int ndirs, const float4* __restrict__ dS,
+                                                       const float4* __restrict__ mie, float4* __restrict__ out,
+                                                       WorkCounters* wc, int entry, float4* s_fold, float3* s_red) {
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
-  const int ih = blockIdx.x / P.nE[1], il = blockIdx.x - ih * P.nE[1];
+  const int ih = entry / P.nE[1], il = entry - ih * P.nE[1];
   const double r0 = index_to_radius(P.g, P.nE[0], ih);
   const double s = index_to_sin_sun_elevation(P.nE[1], il);
   const double c = sqrt(fmax(0.0, 1 - s * s));
@@ -1253,9 +1265,19 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParam
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
       t.x += s_red[w].x; t.y += s_red[w].y; t.z += s_red[w].z;
     }
-    out[blockIdx.x] = make_float4(t.x, t.y, t.z, 0.f);
+    out[entry] = make_float4(t.x, t.y, t.z, 0.f);
   }
   if (threadIdx.x == 0) count(wc, 0, 0, (unsigned long long)ndirs * (mie ? 2 : 1), 0);
+}
+
+template <bool kFold>
+__global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P, int ndirs,
+                                                                  const float4* __restrict__ dS,
+                                                                  const float4* __restrict__ mie,
+                                                                  float4* __restrict__ out, WorkCounters* wc) {
+  __shared__ float3 s_red[kBlock / 64];
+  extern __shared__ float4 s_fold[];  // kFold: [NE x NA] dS, then [NE x NA] Mie strength
+  surface_radiance_entry<kFold>(P, ndirs, dS, mie, out, wc, blockIdx.x, s_fold, s_red);
 }
 
 // f(backward(i)) re-sampling of interpolated tables (interpolate.clj:54-72 with interpolation-table).
@@ -1576,29 +1598,38 @@ size_t direction_geometry_bytes(const DevParams& P, int ndirs) { return sizeof(D
 
 hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
                                 const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
-                                WorkCounters* wc, hipStream_t stream) {
+                                WorkCounters* wc, hipStream_t stream, const SrJob* sr) {
   unsigned combos = (unsigned)(P.nS[0] * ndirs);
   hipLaunchKernelGGL(point_scatter_combine_kernel, dim3(combos), dim3(kBlock), 0, stream, P, (const DirGeom*)geom, ndirs,
                      dS, mie, slab, mie ? slab_mie : nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   const size_t e_bytes = sizeof(float4) * (size_t)P.nE[1];  // dE height row 0, the ground
   const bool e_lds = e_bytes <= 16384;
-  const size_t shmem = sizeof(DirLds) * (size_t)ndirs + (e_lds ? e_bytes : 0);
+  size_t shmem = sizeof(DirLds) * (size_t)ndirs + (e_lds ? e_bytes : 0);
+  SrJob job{};
+  unsigned extra = 0;
+  if (sr) {
+    if (!P.sgeo || !sr->out || !sr_fits_point_scatter(P, sr->mie != nullptr)) return hipErrorInvalidValue;
+    job = *sr;
+    job.count = P.nE[0] * P.nE[1];
+    extra = (unsigned)job.count;
+    shmem = std::max(shmem, sr_fold_bytes(P, sr->mie != nullptr));
+  }
+  const unsigned blocks = (unsigned)(P.nS[0] * P.nS[1]) + extra;
   const DirGeom* g = (const DirGeom*)geom;
   if (mie && e_lds)
-    hipLaunchKernelGGL((point_scatter_kernel<true, true>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs, slab,
-                       slab_mie, dE, out, wc);
+    hipLaunchKernelGGL((point_scatter_kernel<true, true>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
+                       slab, slab_mie, dE, out, job, wc);
   else if (mie)
-    hipLaunchKernelGGL((point_scatter_kernel<true, false>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
-                       slab, slab_mie, dE, out, wc);
+    hipLaunchKernelGGL((point_scatter_kernel<true, false>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
+                       slab, slab_mie, dE, out, job, wc);
   else if (e_lds)
-    hipLaunchKernelGGL((point_scatter_kernel<false, true>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
-                       slab, nullptr, dE, out, wc);
+    hipLaunchKernelGGL((point_scatter_kernel<false, true>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
+                       slab, nullptr, dE, out, job, wc);
   else
-    hipLaunchKernelGGL((point_scatter_kernel<false, false>), dim3(rays), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
-                       slab, nullptr, dE, out, wc);
+    hipLaunchKernelGGL((point_scatter_kernel<false, false>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
+                       slab, nullptr, dE, out, job, wc);
   return hipGetLastError();
 }
 
@@ -1646,11 +1677,15 @@ hipError_t launch_surface_geometry(const DevParams& P, const Dir* dirs, int ndir
   return hipGetLastError();
 }
 
+size_t sr_fold_bytes(const DevParams& P, bool mie) { return sizeof(float4) * (size_t)P.nS[1] * P.nS[3] * (mie ? 2 : 1); }
+
+bool sr_fits_point_scatter(const DevParams& P, bool mie) { return sr_fold_bytes(P, mie) <= 16384; }
+
 hipError_t launch_surface_radiance(const DevParams& P, int ndirs, const float4* dS, const float4* mie, float4* out,
                                    WorkCounters* wc, hipStream_t stream) {
   if (!P.sgeo) return hipErrorInvalidValue;
   unsigned entries = (unsigned)(P.nE[0] * P.nE[1]);
-  const size_t fold = sizeof(float4) * (size_t)P.nS[1] * P.nS[3] * (mie ? 2 : 1);
+  const size_t fold = sr_fold_bytes(P, mie != nullptr);
   if (fold <= 65536)
     hipLaunchKernelGGL(surface_radiance_kernel<true>, dim3(entries), dim3(kBlock), fold, stream, P, ndirs, dS, mie, out,
                        wc);

@@ -23,9 +23,21 @@ size_t direction_geometry_bytes(const DevParams& P, int ndirs);
 // dS: previous-order scatter table; mie (nullable): first-iteration Mie strength table, added with the
 // Mie phase of the quadrature direction; dE: previous-order surface-radiance table.  slab / slab_mie
 // are scratch tables of NH x ndirs x NL x NA float4 (slab_mie used only when mie is given).
+// The surface radiance of the same dS and Mie strength (launch_surface_radiance) computed in the
+// point-scatter launch: ndirs half-sphere directions, output table out; count is set by the launcher.
+struct SrJob {
+  int count;
+  int ndirs;
+  const float4* dS;
+  const float4* mie;
+  float4* out;
+};
+// Whether launch_point_scatter can take the surface radiance along (its LDS slab fits 16 KiB).
+bool sr_fits_point_scatter(const DevParams& P, bool mie);
+size_t sr_fold_bytes(const DevParams& P, bool mie);
 hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
                                 const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
-                                WorkCounters* wc, hipStream_t stream);
+                                WorkCounters* wc, hipStream_t stream, const SrJob* sr = nullptr);
 // The accumulations of atmosphere_lut.clj:95-97 that ride in a ray-scatter launch.  S_out (nullable,
 // with S_in): the running sum S + dS at every grid point, S = S_in, or tab(S_in) when S_resample (S_in
 // then holds the previous iteration's S + dS).  E_out (nullable): E_out = tab(dE + E_in) over the
