@@ -56,10 +56,6 @@ constexpr int kSunRows = 768, kSunCols = 768;
 struct sfsim_atmo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  // a second stream (default priority) and fork/join events: in the captured builds the small per-build
-  // geometry launches run beside the sun table, ray geometry and single scattering
-  hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string error;
   // tables (float4 per entry)
   DeviceBuffer T, dE[2], Eacc[2], R1, M1, dJ, dS[2], Sacc[2];
@@ -525,35 +521,42 @@ int prepare_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
 // capture (nullable): capture[k - 1] receives the sum dS_1 + ... + dS_k of the scattering tables
 // (float4 tables, k = 1..min(iterations, 5)), the training targets of the surrogate.
 // The per-build launches before the iterations: T, E0 and (with iterations) the quadrature directions'
-// geometry, and on the other branch the ray geometry, the re-sampling map and R1/M1.  fork: the first
-// group goes to the side stream, joined back before the function returns (graph capture only; the
-// timed and counted builds stay on one stream).
+// geometry, the ray geometry, the re-sampling map, the sun table, then R1/M1.  merged: the independent
+// ones in one launch (graph capture; the timed and counted builds launch them one by one).
 int enqueue_prologue(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevParams& P, WorkCounters* wc,
-                     Timer& timer, bool fork) {
+                     Timer& timer, bool merged) {
   auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
-  hipStream_t s2 = ctx->stream;
-  if (fork) {
-    HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
-    HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
-    s2 = ctx->side;
+  const bool dirs = p->iterations > 0;
+  if (merged) {
+    PrologueJobs J;
+    J.T = f4(ctx->T);
+    J.E0 = f4(ctx->dE[0]);
+    J.sdirs = dirs ? (const Dir*)ctx->sphere_dirs.ptr : nullptr;
+    J.sphere_n = ctx->sphere_n;
+    J.geom = ctx->geom.ptr;
+    J.hdirs = dirs ? (const Dir*)ctx->half_dirs.ptr : nullptr;
+    J.half_n = ctx->half_n;
+    J.wc = wc;
+    LAUNCH(timer, kRayGeometry, launch_prologue(P, J, ctx->stream));
+    LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream, true));
+    return 0;
   }
-  LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), wc, s2));
-  LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), wc, s2));
-  if (p->iterations > 0) {
+  LAUNCH(timer, kTransmittance, launch_transmittance(P, f4(ctx->T), wc, ctx->stream));
+  LAUNCH(timer, kSurfaceRadianceBase, launch_surface_radiance_base(P, f4(ctx->dE[0]), wc, ctx->stream));
+  if (dirs) {
     LAUNCH(timer, kDirectionGeometry, launch_direction_geometry(P, (const Dir*)ctx->sphere_dirs.ptr, ctx->sphere_n,
-                                                                ctx->geom.ptr, wc, s2));
-    LAUNCH(timer, kDirectionGeometry, launch_surface_geometry(P, (const Dir*)ctx->half_dirs.ptr, ctx->half_n, s2));
+                                                                ctx->geom.ptr, wc, ctx->stream));
+    LAUNCH(timer, kDirectionGeometry,
+           launch_surface_geometry(P, (const Dir*)ctx->half_dirs.ptr, ctx->half_n, ctx->stream));
   }
-  if (fork) HIP_TRY(ctx, hipEventRecord(ctx->join_ev, ctx->side));
   LAUNCH(timer, kRayGeometry, launch_ray_geometry(P, wc, ctx->stream));
   LAUNCH(timer, kAccumulate, launch_resample_map(P, ctx->stream));
   LAUNCH(timer, kSingleScatter, launch_single_scatter(P, f4(ctx->R1), f4(ctx->M1), wc, ctx->stream));
-  if (fork) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
   return 0;
 }
 
 int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevParams& P, WorkCounters* wc,
-                     Timer& timer, float4* const* capture, bool fork = false) {
+                     Timer& timer, float4* const* capture, bool merged = false) {
   const size_t nE = (size_t)count2(p->surface_radiance_shape);
   const size_t nS = (size_t)count4(p->scatter_shape);
   (void)nE;
@@ -561,7 +564,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
   const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
   const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
 
-  TRY(enqueue_prologue(ctx, p, P, wc, timer, fork));
+  TRY(enqueue_prologue(ctx, p, P, wc, timer, merged));
   // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
   // The accumulations E <- table(E + dE), S <- table(S + dS) (atmosphere_lut.clj:95-97) ride in the
   // ray-scatter launches: iteration k writes Ssum[k % 2] = S + dS_k, with S = R1 for the first and
@@ -753,7 +756,7 @@ int prepare_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, const
 // caller's sums uploaded), m applications, the residual and the final tables.  With host_sums == nullptr
 // there is no host copy, allocation or memset, so the sequence can be captured.
 int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevParams& P, int m, const float* host_sums,
-                   Timer& timer, bool fork = false) {
+                   Timer& timer, bool merged = false) {
   const size_t nE = (size_t)count2(p->surface_radiance_shape);
   const size_t nS = (size_t)count4(p->scatter_shape);
   auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
@@ -761,7 +764,7 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
   const long long n4 = (long long)nS, e4 = (long long)nE;
   const int j = kMlpOrders - m;  // seeded partial sum X_j, j >= 1
-  TRY(enqueue_prologue(ctx, p, P, nullptr, timer, fork));
+  TRY(enqueue_prologue(ctx, p, P, nullptr, timer, merged));
   // the network's order sums: P_1..P_4 into pred[0..3] (the ones below the seed feed the final
   // table's re-sampling chain), its P_5 into dS[1] (residual)
   float4* want[kMlpOrders];
@@ -954,13 +957,7 @@ int sfsim_atmo_create(int device, sfsim_atmo_ctx** out) {
   sfsim_atmo_ctx* ctx = new (std::nothrow) sfsim_atmo_ctx();
   if (!ctx) return SFSIM_ATMO_ERR_ALLOC;
   ctx->device = device;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
-    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-    if (ctx->side) (void)hipStreamDestroy(ctx->side);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return SFSIM_ATMO_ERR_DEVICE;
   }
@@ -987,9 +984,6 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   drop_graph(ctx);
   drop_seeded_graph(ctx);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
-  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
-  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;

@@ -294,10 +294,10 @@ __device__ __forceinline__ int xcd_ray(int b, int rays) {
 
 // ------------------------------------------------------------------------------ kernels
 // T table: atmosphere_lut.clj:74 over transmittance-space (atmosphere.clj:291-319)
-__global__ void __launch_bounds__(kBlock) transmittance_kernel(const DevParams P, float4* __restrict__ out,
-                                                               WorkCounters* wc) {
+__device__ __forceinline__ void transmittance_body(const DevParams& P, float4* __restrict__ out,
+                                                               WorkCounters* wc, unsigned blk) {
   const int NH = P.nT[0], NE = P.nT[1];
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  int idx = blk * blockDim.x + threadIdx.x;
   if (idx >= NH * NE) return;
   int ih = idx / NE, ie = idx - ih * NE;
   double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
@@ -311,11 +311,16 @@ __global__ void __launch_bounds__(kBlock) transmittance_kernel(const DevParams P
   count(wc, 0, P.ray_steps, 0, 0);
 }
 
+__global__ void __launch_bounds__(kBlock) transmittance_kernel(const DevParams P, float4* __restrict__ out,
+                                                               WorkCounters* wc) {
+  transmittance_body(P, out, wc, blockIdx.x);
+}
+
 // dE0 table: surface-radiance-base (atmosphere.clj:131-137) over surface-radiance-space
-__global__ void __launch_bounds__(kBlock) surface_radiance_base_kernel(const DevParams P, float4* __restrict__ out,
-                                                                       WorkCounters* wc) {
+__device__ __forceinline__ void surface_radiance_base_body(const DevParams& P, float4* __restrict__ out,
+                                                                       WorkCounters* wc, unsigned blk) {
   const int NH = P.nE[0], NL = P.nE[1];
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  int idx = blk * blockDim.x + threadIdx.x;
   if (idx >= NH * NL) return;
   int ih = idx / NL, il = idx - ih * NL;
   double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
@@ -326,6 +331,11 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_base_kernel(const Dev
   float cosf_ = (float)fmax(0.0, s);  // max(0, normal . l), normal = (1 0 0)
   out[idx] = make_float4(T.x * P.intensity[0] * cosf_, T.y * P.intensity[1] * cosf_, T.z * P.intensity[2] * cosf_, 0.f);
   count(wc, 0, P.ray_steps, 0, 0);
+}
+
+__global__ void __launch_bounds__(kBlock) surface_radiance_base_kernel(const DevParams P, float4* __restrict__ out,
+                                                                       WorkCounters* wc) {
+  surface_radiance_base_body(P, out, wc, blockIdx.x);
 }
 
 // Per-ray setup shared by the 4D kernels: grid point x = (r0 0 0), view v = (se ce 0), ray length D.
@@ -514,9 +524,9 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
 // spans every direction in which the sun is above the horizon.  Entries hold log2 of the node sums
 // (FP64 for grazing segments, compensated FP32 otherwise); interpolation is bilinear in log2
 // (exponential in altitude between grid points).
-__global__ void __launch_bounds__(kBlock) sun_depth_kernel(const DevParams P) {
+__device__ __forceinline__ void sun_depth_body(const DevParams& P, unsigned blk) {
   const int rows = P.sun_rows, cols = P.sun_cols;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = blk * blockDim.x + threadIdx.x;
   if (idx >= rows * cols) return;
   const int ia = idx / cols, ib = idx - ia * cols;
   const double rho = P.g.rho_max * ia / (rows - 1);
@@ -543,13 +553,17 @@ __global__ void __launch_bounds__(kBlock) sun_depth_kernel(const DevParams P) {
   P.sun_depth[idx] = out;
 }
 
+__global__ void __launch_bounds__(kBlock) sun_depth_kernel(const DevParams P) {
+  sun_depth_body(P, blockIdx.x);
+}
+
 // Geometry of every (view ray, sample) pair, one thread each: the sample position, T(x, p_i) over
 // ray_steps midpoint nodes, the height and elevation coordinates of the dJ lookup, the terms of the
 // sun-elevation coordinate and, with a sun table, the sample's row and exit-distance terms in it.
 // Shared by the single-scatter kernel and the ray-scatter launches of all iterations.
-__global__ void __launch_bounds__(kBlock) ray_geometry_kernel(const DevParams P, WorkCounters* wc) {
+__device__ __forceinline__ void ray_geometry_body(const DevParams& P, WorkCounters* wc, unsigned blk) {
   const int n = P.ray_steps, NH = P.nS[0], NE = P.nS[1];
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = (long long)blk * blockDim.x + threadIdx.x;
   if (g >= (long long)NH * NE * n) return;
   const int rayid = (int)(g / n), i = (int)(g - (long long)rayid * n);
   const int ih = rayid / NE, ie = rayid - ih * NE;
@@ -575,6 +589,10 @@ __global__ void __launch_bounds__(kBlock) ray_geometry_kernel(const DevParams P,
     V.row[g] = make_float2(x - row, __int_as_float((int)row * cols));
   }
   count(wc, 0, n, 0, 0);
+}
+
+__global__ void __launch_bounds__(kBlock) ray_geometry_kernel(const DevParams P, WorkCounters* wc) {
+  ray_geometry_body(P, wc, blockIdx.x);
 }
 
 // Single scatter with the sun segments' optical depths read from the sun_depth table: one block per
@@ -942,12 +960,12 @@ struct DirGeom {
   float4 phat;    // P / |P| (w unused: P is on the ground, height row 0 of the surface-radiance table)
 };
 
-__global__ void __launch_bounds__(kBlock) direction_geometry_kernel(const DevParams P, const Dir* __restrict__ dirs,
+__device__ __forceinline__ void direction_geometry_body(const DevParams& P, const Dir* __restrict__ dirs,
                                                                     int ndirs, DirGeom* __restrict__ geom,
-                                                                    WorkCounters* wc) {
+                                                                    WorkCounters* wc, unsigned blk) {
   const int NH = P.nS[0], NE = P.nS[1];
   const int n = P.ray_steps;
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  int idx = blk * blockDim.x + threadIdx.x;
   bool active = idx < NH * ndirs;
   int ih = active ? idx / ndirs : 0, j = active ? idx - ih * ndirs : 0;
   const double r0 = index_to_radius(P.g, NH, ih), h0 = index_to_altitude(P.g, NH, ih);
@@ -975,6 +993,12 @@ __global__ void __launch_bounds__(kBlock) direction_geometry_kernel(const DevPar
   }
   geom[idx] = g;
   count(wc, 0, surface ? n : 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(kBlock) direction_geometry_kernel(const DevParams P, const Dir* __restrict__ dirs,
+                                                                    int ndirs, DirGeom* __restrict__ geom,
+                                                                    WorkCounters* wc) {
+  direction_geometry_body(P, dirs, ndirs, geom, wc, blockIdx.x);
 }
 
 // slab[h][j][s][a] = dS at (height of x_h, elevation coordinate of omega_j) for every (s, a) grid
@@ -1181,15 +1205,20 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
 // dE' = surface-radiance of dS (atmosphere_lut.clj:89,93; atmosphere.clj:225-230): one workgroup per
 // entry of the surface-radiance table, lanes over half-sphere directions, workgroup reduction.
 // The half-sphere directions' geometry at every surface-radiance height (once per build).
-__global__ void __launch_bounds__(kBlock) surface_geometry_kernel(const DevParams P, const Dir* __restrict__ dirs,
-                                                                  int ndirs) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void surface_geometry_body(const DevParams& P, const Dir* __restrict__ dirs,
+                                                                  int ndirs, unsigned blk) {
+  const int idx = blk * blockDim.x + threadIdx.x;
   if (idx >= P.nE[0] * ndirs) return;
   const int ih = idx / ndirs, j = idx - ih * ndirs;
   const double r0 = index_to_radius(P.g, P.nE[0], ih);
   const Dir d = dirs[j];
   P.sgeo[idx] = make_float4((float)d.x, (float)d.y, (float)elevation_to_index(P.g, P.nS[1], r0, r0 * d.x, true),
                             (float)(d.x * d.w));  // (omega . normal) * quadrature weight, normal = (1 0 0)
+}
+
+__global__ void __launch_bounds__(kBlock) surface_geometry_kernel(const DevParams P, const Dir* __restrict__ dirs,
+                                                                  int ndirs) {
+  surface_geometry_body(P, dirs, ndirs, blockIdx.x);
 }
 
 // kFold: the entry's height and sun-elevation coordinates are the same for every direction, so the
@@ -1307,10 +1336,10 @@ __device__ __forceinline__ Lerp lerp_of(int i0, float s, int N) {
 }
 
 // The 4D re-sampling lookups of a build, once (every accumulation and final table reads them).
-__global__ void __launch_bounds__(kBlock) resample_map_kernel(const DevParams P) {
+__device__ __forceinline__ void resample_map_body(const DevParams& P, unsigned blk) {
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const long long n = (long long)NH * NE * NL * NA;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long idx = (long long)blk * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   const int ia = (int)(idx % NA);
   long long q = idx / NA;
@@ -1322,6 +1351,36 @@ __global__ void __launch_bounds__(kBlock) resample_map_kernel(const DevParams P)
   ((float4*)P.rmap)[idx] = make_float4(Lh.s, Le.s, Ls.s, La.s);
   ((ushort4*)(P.rmap + n * sizeof(float4)))[idx] =
       make_ushort4((unsigned short)Lh.i0, (unsigned short)Le.i0, (unsigned short)Ls.i0, (unsigned short)La.i0);
+}
+
+__global__ void __launch_bounds__(kBlock) resample_map_kernel(const DevParams P) {
+  resample_map_body(P, blockIdx.x);
+}
+
+// The per-build jobs in one launch, job k over workgroups first[k] .. first[k + 1] - 1, the small
+// long-running grids first so that they start at once: direction geometry, transmittance, E0, surface
+// geometry, the sun table, ray geometry, the re-sampling map.  An absent job has an empty range.
+struct PrologueArgs {
+  PrologueJobs j;
+  unsigned first[8];
+};
+
+__global__ void __launch_bounds__(kBlock) prologue_kernel(const DevParams P, const PrologueArgs A) {
+  const unsigned b = blockIdx.x;
+  if (b < A.first[1])
+    direction_geometry_body(P, A.j.sdirs, A.j.sphere_n, (DirGeom*)A.j.geom, A.j.wc, b - A.first[0]);
+  else if (b < A.first[2])
+    transmittance_body(P, A.j.T, A.j.wc, b - A.first[1]);
+  else if (b < A.first[3])
+    surface_radiance_base_body(P, A.j.E0, A.j.wc, b - A.first[2]);
+  else if (b < A.first[4])
+    surface_geometry_body(P, A.j.hdirs, A.j.half_n, b - A.first[3]);
+  else if (b < A.first[5])
+    sun_depth_body(P, b - A.first[4]);
+  else if (b < A.first[6])
+    ray_geometry_body(P, A.j.wc, b - A.first[5]);
+  else
+    resample_map_body(P, b - A.first[6]);
 }
 
 // Value at 4D entry idx of tab(a + b) (b nullable): the multilinear lookup at forward(backward(idx)),
@@ -1565,11 +1624,13 @@ hipError_t launch_ray_geometry(const DevParams& P, WorkCounters* wc, hipStream_t
 }
 
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
-                                 hipStream_t stream) {
+                                 hipStream_t stream, bool sun_table_built) {
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   if (P.sun_rows > 0) {
     if (!P.sun_depth || !P.ray_geo || P.sun_rows < 2 || P.sun_cols < 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(sun_depth_kernel, dim3(blocks_for((long long)P.sun_rows * P.sun_cols)), dim3(kBlock), 0, stream, P);
+    if (!sun_table_built)
+      hipLaunchKernelGGL(sun_depth_kernel, dim3(blocks_for((long long)P.sun_rows * P.sun_cols)), dim3(kBlock), 0, stream,
+                         P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const RayGeoView V = ray_geo_view(P);
@@ -1691,6 +1752,24 @@ hipError_t launch_surface_radiance(const DevParams& P, int ndirs, const float4* 
                        wc);
   else
     hipLaunchKernelGGL(surface_radiance_kernel<false>, dim3(entries), dim3(kBlock), 0, stream, P, ndirs, dS, mie, out, wc);
+  return hipGetLastError();
+}
+
+hipError_t launch_prologue(const DevParams& P, const PrologueJobs& J, hipStream_t stream) {
+  if (!P.ray_geo || !P.rmap || (P.sun_rows > 0 && !P.sun_depth)) return hipErrorInvalidValue;
+  if ((J.sdirs && !J.geom) || (J.hdirs && !P.sgeo) || !J.T || !J.E0) return hipErrorInvalidValue;
+  PrologueArgs A;
+  A.j = J;
+  const long long counts[7] = {J.sdirs ? (long long)P.nS[0] * J.sphere_n : 0,
+                               (long long)P.nT[0] * P.nT[1],
+                               (long long)P.nE[0] * P.nE[1],
+                               J.hdirs ? (long long)P.nE[0] * J.half_n : 0,
+                               P.sun_rows > 0 ? (long long)P.sun_rows * P.sun_cols : 0,
+                               (long long)ray_geo_samples(P),
+                               (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3]};
+  A.first[0] = 0;
+  for (int k = 0; k < 7; ++k) A.first[k + 1] = A.first[k] + blocks_for(counts[k]);
+  hipLaunchKernelGGL(prologue_kernel, dim3(A.first[7]), dim3(kBlock), 0, stream, P, A);
   return hipGetLastError();
 }
 

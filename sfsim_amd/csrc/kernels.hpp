@@ -12,9 +12,24 @@ hipError_t launch_transmittance(const DevParams& P, float4* out, WorkCounters* w
 hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream);
 // Per-(view ray, sample) geometry into P.ray_geo (once per build, before single and ray scatter).
 hipError_t launch_ray_geometry(const DevParams& P, WorkCounters* wc, hipStream_t stream);
-// R1 and M1; with a sun table (P.sun_rows > 0) builds it first and reads P.ray_geo.
+// R1 and M1; with a sun table (P.sun_rows > 0) builds it first (unless sun_table_built: launch_prologue
+// did) and reads P.ray_geo.
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
-                                 hipStream_t stream);
+                                 hipStream_t stream, bool sun_table_built = false);
+// The per-build launches that read nothing of each other, in one launch: direction geometry of the
+// sphere quadrature (when sdirs, into geom), transmittance T, E0, the half-sphere geometry (when hdirs,
+// into P.sgeo), the sun table (P.sun_rows > 0), the ray geometry and the re-sampling map.
+struct PrologueJobs {
+  float4* T;
+  float4* E0;
+  const Dir* sdirs;
+  int sphere_n;
+  void* geom;
+  const Dir* hdirs;
+  int half_n;
+  WorkCounters* wc;
+};
+hipError_t launch_prologue(const DevParams& P, const PrologueJobs& J, hipStream_t stream);
 // Per-(height, direction) geometry of the sphere quadrature (once per build); geom holds
 // direction_geometry_bytes(P, ndirs) bytes.
 hipError_t launch_direction_geometry(const DevParams& P, const Dir* dirs, int ndirs, void* geom, WorkCounters* wc,
