@@ -74,6 +74,7 @@ struct sfsim_atmo_ctx {
   DeviceBuffer sun_depth;
   DeviceBuffer ray_geo;  // per-(view ray, sample) geometry of single and ray scatter
   DeviceBuffer rmap;     // per-entry 4D re-sampling lookups of the pipelines
+  DeviceBuffer sunmap;   // per-entry sun terms of the point- and ray-scatter kernels (with rmap)
   DeviceBuffer sgeo;     // half-sphere direction geometry per surface-radiance height
   DeviceBuffer ray_order;  // ray-scatter visiting order (ascending impact parameter)
   DeviceBuffer Ssum[2];    // S + dS at the grid points, written by ray scatter (alternate iterations)
@@ -305,6 +306,10 @@ DevParams dev_params(const sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   }
   P.ray_geo = (char*)ctx->ray_geo.ptr;
   P.rmap = (char*)ctx->rmap.ptr;  // filled by launch_resample_map at the start of a pipeline
+  if (ctx->sunmap.ptr) {          // filled with rmap
+    P.sun_ps = (float4*)ctx->sunmap.ptr;
+    P.sun_rs = P.sun_ps + count4(p->scatter_shape);
+  }
   P.sgeo = (float4*)ctx->sgeo.ptr;  // filled by launch_surface_geometry before the surface-radiance stages
   P.ray_order = (const int*)ctx->ray_order.ptr;  // set by ensure_ray_order
   return P;
@@ -354,6 +359,7 @@ int ensure_scatter_buffers(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(ensure_sun_depth(ctx));
   TRY(ensure_ray_order(ctx, p));
   TRY(ensure(ctx, ctx->rmap, resample_map_bytes(make_dev_params(p))));
+  TRY(ensure(ctx, ctx->sunmap, 2 * sizeof(float4) * (size_t)count4(p->scatter_shape)));
   return ensure(ctx, ctx->ray_geo, ray_geometry_bytes(make_dev_params(p)));
 }
 
@@ -978,7 +984,7 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum[0], &ctx->Ssum[1]};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sunmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum[0], &ctx->Ssum[1]};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
@@ -1061,6 +1067,7 @@ int sfsim_atmo_point_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params*
   if (!dS || !dE || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dS, dE and out must not be NULL");
   TRY(ensure_dirs(ctx, 0, p->sphere_steps));
   DevParams P = dev_params(ctx, p);
+  P.sun_ps = P.sun_rs = nullptr;  // no re-sampling map in a single stage: the kernel computes the sun terms
   size_t n = (size_t)count4(p->scatter_shape), ne = (size_t)count2(p->surface_radiance_shape);
   const float4 *ddS, *dM, *ddE;
   TRY(upload_table(ctx, 0, dS, n, &ddS));
@@ -1081,6 +1088,7 @@ int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p
   if (!dJ || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dJ and out must not be NULL");
   TRY(ensure_scatter_buffers(ctx, p));
   DevParams P = dev_params(ctx, p);
+  P.sun_ps = P.sun_rs = nullptr;  // no re-sampling map in a single stage: the kernel computes the sun terms
   size_t n = (size_t)count4(p->scatter_shape);
   const float4* ddJ;
   TRY(upload_table(ctx, 0, dJ, n, &ddJ));

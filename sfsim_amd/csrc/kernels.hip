@@ -898,13 +898,20 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     xl[q] = 0.f;
     La[q] = make_lerp(0.f, NA);
     if (k < NLA) {
-      int is = k / NA, ia = k - is * NA;
-      double sin_s = index_to_sin_sun_elevation(NL, is);
-      double l0, l1, l2;
-      index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
-      vl[q] = (float)(ray.se * l0 + ray.ce * l1);
-      xl[q] = (float)(ray.r0 * l0);
-      La[q] = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
+      if (P.sun_rs) {  // the per-build sun terms
+        const float4 m = P.sun_rs[(long long)rayid * NLA + k];
+        vl[q] = m.x;
+        xl[q] = m.y;
+        La[q] = make_lerp(m.z, NA);
+      } else {
+        int is = k / NA, ia = k - is * NA;
+        double sin_s = index_to_sin_sun_elevation(NL, is);
+        double l0, l1, l2;
+        index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
+        vl[q] = (float)(ray.se * l0 + ray.ce * l1);
+        xl[q] = (float)(ray.r0 * l0);
+        La[q] = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
+      }
     }
   }
   __syncthreads();  // chunk 0 folded
@@ -1182,12 +1189,24 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   const long long base = (long long)rayid * NLA;
   unsigned long long l4 = 0, l2 = 0;
   for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
-    int is = k / NA, ia = k - is * NA;
-    double sin_s = index_to_sin_sun_elevation(NL, is);
-    double l0, l1, l2d;
-    index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2d);
-    const float lx = (float)l0, ly = (float)l1, lz = (float)l2d;
-    const Lerp Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
+    float lx, ly, lz;
+    Lerp Ls;
+    if (P.sun_ps) {  // the per-build sun terms
+      const float4 m = P.sun_ps[base + k];
+      lx = m.x;
+      ly = m.y;
+      lz = m.z;
+      Ls = make_lerp(m.w, NL);
+    } else {
+      int is = k / NA, ia = k - is * NA;
+      double sin_s = index_to_sin_sun_elevation(NL, is);
+      double l0, l1, l2d;
+      index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2d);
+      lx = (float)l0;
+      ly = (float)l1;
+      lz = (float)l2d;
+      Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
+    }
     const float row_bytes = (float)(Ls.i0 * NA * (int)sizeof(float4));
     // a grid entry's own sun-elevation coordinate is an integer, so its weight is exactly zero and only
     // one row of each slab is read; the general lookup stays for safety
@@ -1311,8 +1330,10 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParam
 
 // f(backward(i)) re-sampling of interpolated tables (interpolate.clj:54-72 with interpolation-table).
 // Lerps of forward(backward(i)) for 4D entry i = ((ih NE + ie) NL + is) NA + ia (atmosphere.clj:233-422).
+// With ps / rs (nullable) also the entry's sun terms that the point- and ray-scatter kernels use
+// (DevParams::sun_ps, sun_rs), from the same FP64 values.
 __device__ __forceinline__ void resample4_lerps(const DevParams& P, int ih, int ie, int is, int ia, Lerp& Lh, Lerp& Le,
-                                                Lerp& Ls, Lerp& La) {
+                                                Lerp& Ls, Lerp& La, float4* ps = nullptr, float4* rs = nullptr) {
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   double r0 = index_to_radius(P.g, NH, ih);
   double se, ce;
@@ -1321,10 +1342,14 @@ __device__ __forceinline__ void resample4_lerps(const DevParams& P, int ih, int 
   double sin_s = index_to_sin_sun_elevation(NL, is);
   double l0, l1, l2;
   index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2);
+  const double cs = sun_elevation_to_index(NL, (r0 * l0) / r0), vl = se * l0 + ce * l1;
+  const double ca = sun_angle_to_index(NA, vl);
   Lh = make_lerp((float)height_to_index(P.g, NH, r0), NH);
   Le = make_lerp((float)elevation_to_index(P.g, NE, r0, r0 * se, above), NE);
-  Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
-  La = make_lerp((float)sun_angle_to_index(NA, se * l0 + ce * l1), NA);
+  Ls = make_lerp((float)cs, NL);
+  La = make_lerp((float)ca, NA);
+  if (ps) *ps = make_float4((float)l0, (float)l1, (float)l2, (float)cs);
+  if (rs) *rs = make_float4((float)vl, (float)(r0 * l0), (float)ca, 0.f);
 }
 
 __device__ __forceinline__ Lerp lerp_of(int i0, float s, int N) {
@@ -1347,8 +1372,13 @@ __device__ __forceinline__ void resample_map_body(const DevParams& P, unsigned b
   q /= NL;
   const int ie = (int)(q % NE), ih = (int)(q / NE);
   Lerp Lh, Le, Ls, La;
-  resample4_lerps(P, ih, ie, is, ia, Lh, Le, Ls, La);
+  float4 ps, rs;
+  resample4_lerps(P, ih, ie, is, ia, Lh, Le, Ls, La, &ps, &rs);
   ((float4*)P.rmap)[idx] = make_float4(Lh.s, Le.s, Ls.s, La.s);
+  if (P.sun_ps) {
+    P.sun_ps[idx] = ps;
+    P.sun_rs[idx] = rs;
+  }
   ((ushort4*)(P.rmap + n * sizeof(float4)))[idx] =
       make_ushort4((unsigned short)Lh.i0, (unsigned short)Le.i0, (unsigned short)Ls.i0, (unsigned short)La.i0);
 }
