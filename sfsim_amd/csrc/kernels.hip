@@ -807,6 +807,7 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
 
 __device__ float3 resample4_value(const DevParams& P, const float4* __restrict__ a, const float4* __restrict__ b,
                                  long long idx);
+__device__ float3 resample4_value_lean(const DevParams& P, const float4* __restrict__ a, long long idx);
 __device__ float3 resample2_value(const DevParams& P, int space, const float4* __restrict__ a,
                                  const float4* __restrict__ b, long long idx);
 
@@ -945,7 +946,7 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
       out[base + k] = d;
       if (accum.S_out) {  // the running sum S + dS of atmosphere_lut.clj:95-97 at this grid point
         const float3 u =
-            accum.S_resample ? resample4_value(P, accum.S_in, nullptr, base + k) : ld3(accum.S_in, base + k);
+            accum.S_resample ? resample4_value_lean(P, accum.S_in, base + k) : ld3(accum.S_in, base + k);
         accum.S_out[base + k] = make_float4(u.x + d.x, u.y + d.y, u.z + d.z, 0.f);
       }
       lookups += n;
@@ -1451,6 +1452,36 @@ __device__ float3 resample4_value(const DevParams& P, const float4* __restrict__
     }
   }
   return v;
+}
+
+// resample4_value(P, a, nullptr, idx) with the per-build map, in the same order of operations, one
+// (height, elevation) cell of four reads at a time: scheduling barriers keep the compiler from issuing
+// all sixteen reads at once, which needs registers that a kernel held at 64 VGPRs does not have.
+__device__ float3 resample4_value_lean(const DevParams& P, const float4* __restrict__ a, long long idx) {
+  const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
+  const long long n = (long long)NH * NE * NL * NA;
+  const float4 w = ((const float4*)P.rmap)[idx];
+  const ushort4 i = ((const ushort4*)(P.rmap + n * sizeof(float4)))[idx];
+  const Lerp Lh = lerp_of(i.x, w.x, NH), Le = lerp_of(i.y, w.y, NE), Ls = lerp_of(i.z, w.z, NL),
+             La = lerp_of(i.w, w.w, NA);
+  // entry offsets in 32 bits: a 4D table has fewer than 2^29 entries (validated on the host)
+  if (Lh.s == 0.0f && Le.s == 0.0f && Ls.s == 0.0f && La.s == 0.0f)
+    return ld3(a, (unsigned)(((Lh.i0 * NE + Le.i0) * NL + Ls.i0) * NA + La.i0));
+  auto cell = [&](int h, int e) {
+    const unsigned bb = (unsigned)((h * NE + e) * NL);
+    const float4* c0 = a + (bb + (unsigned)Ls.i0) * (unsigned)NA;
+    const float4* c1 = a + (bb + (unsigned)Ls.i1) * (unsigned)NA;
+    const float3 u = mix3(ld3(c0, La.i0), ld3(c0, La.i1), La.s);
+    const float3 v = mix3(ld3(c1, La.i0), ld3(c1, La.i1), La.s);
+    const float3 r = mix3(u, v, Ls.s);
+    __builtin_amdgcn_sched_barrier(0);
+    return r;
+  };
+  const float3 r00 = cell(Lh.i0, Le.i0);
+  const float3 r0 = mix3(r00, cell(Lh.i0, Le.i1), Le.s);
+  const float3 r10 = cell(Lh.i1, Le.i0);
+  const float3 r1 = mix3(r10, cell(Lh.i1, Le.i1), Le.s);
+  return mix3(r0, r1, Lh.s);
 }
 
 // Value at 2D entry idx of tab(a + b) (b nullable) for the transmittance (space 2) or
