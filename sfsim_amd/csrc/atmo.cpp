@@ -1066,8 +1066,8 @@ int sfsim_atmo_point_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params*
   TRY(validate_sphere(ctx, p->sphere_steps));
   if (!dS || !dE || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dS, dE and out must not be NULL");
   TRY(ensure_dirs(ctx, 0, p->sphere_steps));
+  TRY(ensure_scatter_buffers(ctx, p));
   DevParams P = dev_params(ctx, p);
-  P.sun_ps = P.sun_rs = nullptr;  // no re-sampling map in a single stage: the kernel computes the sun terms
   size_t n = (size_t)count4(p->scatter_shape), ne = (size_t)count2(p->surface_radiance_shape);
   const float4 *ddS, *dM, *ddE;
   TRY(upload_table(ctx, 0, dS, n, &ddS));
@@ -1077,6 +1077,7 @@ int sfsim_atmo_point_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params*
   float4* o = stage_out(ctx, n, &rc);
   TRY(rc);
   TRY(prepare_point_scatter(ctx, P, dM != nullptr, nullptr));
+  HIP_TRY(ctx, launch_resample_map(P, ctx->stream));  // with it the per-entry sun terms
   HIP_TRY(ctx, launch_point_scatter(P, (const Dir*)ctx->sphere_dirs.ptr, ctx->sphere_n, ctx->geom.ptr, ddS, dM, ddE,
                                     (float4*)ctx->slab.ptr, (float4*)ctx->slab_mie.ptr, o, nullptr, ctx->stream));
   return download_table(ctx, o, out, n);
@@ -1088,7 +1089,6 @@ int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p
   if (!dJ || !out) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "dJ and out must not be NULL");
   TRY(ensure_scatter_buffers(ctx, p));
   DevParams P = dev_params(ctx, p);
-  P.sun_ps = P.sun_rs = nullptr;  // no re-sampling map in a single stage: the kernel computes the sun terms
   size_t n = (size_t)count4(p->scatter_shape);
   const float4* ddJ;
   TRY(upload_table(ctx, 0, dJ, n, &ddJ));
@@ -1096,6 +1096,7 @@ int sfsim_atmo_ray_scatter_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p
   float4* o = stage_out(ctx, n, &rc);
   TRY(rc);
   HIP_TRY(ctx, launch_ray_geometry(P, nullptr, ctx->stream));
+  HIP_TRY(ctx, launch_resample_map(P, ctx->stream));  // with it the per-entry sun terms
   HIP_TRY(ctx, launch_ray_scatter(P, ddJ, o, nullptr, ctx->stream));
   return download_table(ctx, o, out, n);
 }

@@ -48,9 +48,10 @@ struct DevParams {
   // per-entry lookup of the 4D re-sampling (forward(backward(i))): NH x NE x NL x NA float4 weights
   // followed by as many ushort4 lower indices (resample_map_bytes); null: computed per launch
   char* rmap;
-  // per-entry sun terms, written with the re-sampling map (null: computed per launch): sun_ps = (l, the
-  // sun-elevation coordinate) for point scattering, sun_rs = (v . l, x . l, the heading coordinate, 0)
-  // for ray scattering, each rounded once from FP64 (NH x NE x NL x NA float4 each)
+  // per-entry sun terms, written with the re-sampling map (launch_resample_map) and read by every
+  // point- and ray-scatter launch: sun_ps = (l, the sun-elevation coordinate) for point scattering,
+  // sun_rs = (v . l, x . l, the heading coordinate, 0) for ray scattering, each rounded once from FP64
+  // (NH x NE x NL x NA float4 each)
   float4* sun_ps;
   float4* sun_rs;
   // per (surface-radiance height, half-sphere direction): omega.x, omega.y, elevation coordinate of

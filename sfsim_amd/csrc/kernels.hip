@@ -899,20 +899,11 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     xl[q] = 0.f;
     La[q] = make_lerp(0.f, NA);
     if (k < NLA) {
-      if (P.sun_rs) {  // the per-build sun terms
-        const float4 m = P.sun_rs[(long long)rayid * NLA + k];
-        vl[q] = m.x;
-        xl[q] = m.y;
-        La[q] = make_lerp(m.z, NA);
-      } else {
-        int is = k / NA, ia = k - is * NA;
-        double sin_s = index_to_sin_sun_elevation(NL, is);
-        double l0, l1, l2;
-        index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
-        vl[q] = (float)(ray.se * l0 + ray.ce * l1);
-        xl[q] = (float)(ray.r0 * l0);
-        La[q] = make_lerp((float)sun_angle_to_index(NA, ray.se * l0 + ray.ce * l1), NA);
-      }
+      // the per-build sun terms (v . l, x . l, the heading coordinate, rounded once from FP64)
+      const float4 m = P.sun_rs[(long long)rayid * NLA + k];
+      vl[q] = m.x;
+      xl[q] = m.y;
+      La[q] = make_lerp(m.z, NA);
     }
   }
   __syncthreads();  // chunk 0 folded
@@ -1192,21 +1183,12 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
   for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
     float lx, ly, lz;
     Lerp Ls;
-    if (P.sun_ps) {  // the per-build sun terms
+    {  // the per-build sun terms (l and the sun-elevation coordinate, rounded once from FP64)
       const float4 m = P.sun_ps[base + k];
       lx = m.x;
       ly = m.y;
       lz = m.z;
       Ls = make_lerp(m.w, NL);
-    } else {
-      int is = k / NA, ia = k - is * NA;
-      double sin_s = index_to_sin_sun_elevation(NL, is);
-      double l0, l1, l2d;
-      index_to_sun_direction(NA, se, ce, sin_s, ia, l0, l1, l2d);
-      lx = (float)l0;
-      ly = (float)l1;
-      lz = (float)l2d;
-      Ls = make_lerp((float)sun_elevation_to_index(NL, (r0 * l0) / r0), NL);
     }
     const float row_bytes = (float)(Ls.i0 * NA * (int)sizeof(float4));
     // a grid entry's own sun-elevation coordinate is an integer, so its weight is exactly zero and only
@@ -1721,6 +1703,7 @@ size_t direction_geometry_bytes(const DevParams& P, int ndirs) { return sizeof(D
 hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, const void* geom, const float4* dS,
                                 const float4* mie, const float4* dE, float4* slab, float4* slab_mie, float4* out,
                                 WorkCounters* wc, hipStream_t stream, const SrJob* sr) {
+  if (!P.sun_ps) return hipErrorInvalidValue;
   unsigned combos = (unsigned)(P.nS[0] * ndirs);
   hipLaunchKernelGGL(point_scatter_combine_kernel, dim3(combos), dim3(kBlock), 0, stream, P, (const DirGeom*)geom, ndirs,
                      dS, mie, slab, mie ? slab_mie : nullptr);
@@ -1772,7 +1755,7 @@ int ray_scatter_chunk(const DevParams& P) {
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream, const RsAccum& acc) {
   if ((acc.S_in == nullptr) != (acc.S_out == nullptr) || (acc.E_out && !acc.dE)) return hipErrorInvalidValue;
-  if (!P.ray_geo) return hipErrorInvalidValue;
+  if (!P.ray_geo || !P.sun_rs) return hipErrorInvalidValue;
   const unsigned rays = (unsigned)(P.nS[0] * P.nS[1]) + (acc.E_out ? 1u : 0u);
   int nla = P.nS[2] * P.nS[3];
   int Q = ray_scatter_chunk(P);
