@@ -309,6 +309,7 @@ DevParams dev_params(const sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   if (ctx->sunmap.ptr) {          // filled with rmap
     P.sun_ps = (float4*)ctx->sunmap.ptr;
     P.sun_rs = P.sun_ps + count4(p->scatter_shape);
+    P.sun_ss = (double2*)(P.sun_rs + count4(p->scatter_shape));
   }
   P.sgeo = (float4*)ctx->sgeo.ptr;  // filled by launch_surface_geometry before the surface-radiance stages
   P.ray_order = (const int*)ctx->ray_order.ptr;  // set by ensure_ray_order
@@ -359,7 +360,7 @@ int ensure_scatter_buffers(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(ensure_sun_depth(ctx));
   TRY(ensure_ray_order(ctx, p));
   TRY(ensure(ctx, ctx->rmap, resample_map_bytes(make_dev_params(p))));
-  TRY(ensure(ctx, ctx->sunmap, 2 * sizeof(float4) * (size_t)count4(p->scatter_shape)));
+  TRY(ensure(ctx, ctx->sunmap, (2 * sizeof(float4) + sizeof(double2)) * (size_t)count4(p->scatter_shape)));
   return ensure(ctx, ctx->ray_geo, ray_geometry_bytes(make_dev_params(p)));
 }
 
@@ -1054,6 +1055,7 @@ int sfsim_atmo_single_scatter_tables(sfsim_atmo_ctx* ctx, const sfsim_atmo_param
   TRY(ensure(ctx, ctx->M1, n * sizeof(float4)));
   ctx->built = false;
   HIP_TRY(ctx, launch_ray_geometry(P, nullptr, ctx->stream));
+  HIP_TRY(ctx, launch_resample_map(P, ctx->stream));  // with it the per-entry sun terms
   HIP_TRY(ctx, launch_single_scatter(P, (float4*)ctx->R1.ptr, (float4*)ctx->M1.ptr, nullptr, ctx->stream));
   TRY(download_table(ctx, (const float4*)ctx->R1.ptr, rayleigh_out, n));
   return download_table(ctx, (const float4*)ctx->M1.ptr, mie_out, n);

@@ -51,9 +51,11 @@ struct DevParams {
   // per-entry sun terms, written with the re-sampling map (launch_resample_map) and read by every
   // point- and ray-scatter launch: sun_ps = (l, the sun-elevation coordinate) for point scattering,
   // sun_rs = (v . l, x . l, the heading coordinate, 0) for ray scattering, each rounded once from FP64
-  // (NH x NE x NL x NA float4 each)
+  // (NH x NE x NL x NA float4 each); sun_ss = (v . l, x . l) in FP64 for single scattering's horizon
+  // tests (the table path)
   float4* sun_ps;
   float4* sun_rs;
+  double2* sun_ss;
   // per (surface-radiance height, half-sphere direction): omega.x, omega.y, elevation coordinate of
   // omega at the height's grid point, cosine-weighted quadrature weight (nE[0] x directions float4)
   float4* sgeo;

@@ -628,12 +628,9 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
   const float2* __restrict__ tab = P.sun_depth;
   unsigned long long lit = 0;
   for (int k = threadIdx.x; k < NLA; k += blockDim.x) {
-    const int is = k / NA, ia = k - is * NA;
-    const double sin_s = index_to_sin_sun_elevation(NL, is);
-    double l0, l1, l2;
-    index_to_sun_direction(NA, ray.se, ray.ce, sin_s, ia, l0, l1, l2);
-    const double vl = ray.se * l0 + ray.ce * l1;
-    const double xl = ray.r0 * l0;
+    // dot(v, l) and dot(x, l) in FP64 from the per-build sun map (the same FP64 evaluation)
+    const double2 vx = P.sun_ss[(long long)rayid * NLA + k];
+    const double vl = vx.x, xl = vx.y;
     float3 accR = make_float3(0.f, 0.f, 0.f), accM = make_float3(0.f, 0.f, 0.f);
     // branch-free: a sample with the sun below the horizon (atmosphere.clj:154-160) reads the table at
     // clamped, in-range coordinates and adds exactly zero, so consecutive samples' reads overlap
@@ -1316,7 +1313,8 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParam
 // With ps / rs (nullable) also the entry's sun terms that the point- and ray-scatter kernels use
 // (DevParams::sun_ps, sun_rs), from the same FP64 values.
 __device__ __forceinline__ void resample4_lerps(const DevParams& P, int ih, int ie, int is, int ia, Lerp& Lh, Lerp& Le,
-                                                Lerp& Ls, Lerp& La, float4* ps = nullptr, float4* rs = nullptr) {
+                                                Lerp& Ls, Lerp& La, float4* ps = nullptr, float4* rs = nullptr,
+                                                double2* ss = nullptr) {
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   double r0 = index_to_radius(P.g, NH, ih);
   double se, ce;
@@ -1333,6 +1331,7 @@ __device__ __forceinline__ void resample4_lerps(const DevParams& P, int ih, int 
   La = make_lerp((float)ca, NA);
   if (ps) *ps = make_float4((float)l0, (float)l1, (float)l2, (float)cs);
   if (rs) *rs = make_float4((float)vl, (float)(r0 * l0), (float)ca, 0.f);
+  if (ss) *ss = make_double2(vl, r0 * l0);
 }
 
 __device__ __forceinline__ Lerp lerp_of(int i0, float s, int N) {
@@ -1356,11 +1355,13 @@ __device__ __forceinline__ void resample_map_body(const DevParams& P, unsigned b
   const int ie = (int)(q % NE), ih = (int)(q / NE);
   Lerp Lh, Le, Ls, La;
   float4 ps, rs;
-  resample4_lerps(P, ih, ie, is, ia, Lh, Le, Ls, La, &ps, &rs);
+  double2 ss;
+  resample4_lerps(P, ih, ie, is, ia, Lh, Le, Ls, La, &ps, &rs, &ss);
   ((float4*)P.rmap)[idx] = make_float4(Lh.s, Le.s, Ls.s, La.s);
   if (P.sun_ps) {
     P.sun_ps[idx] = ps;
     P.sun_rs[idx] = rs;
+    P.sun_ss[idx] = ss;
   }
   ((ushort4*)(P.rmap + n * sizeof(float4)))[idx] =
       make_ushort4((unsigned short)Lh.i0, (unsigned short)Le.i0, (unsigned short)Ls.i0, (unsigned short)La.i0);
@@ -1670,7 +1671,7 @@ hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* m
                                  hipStream_t stream, bool sun_table_built) {
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
   if (P.sun_rows > 0) {
-    if (!P.sun_depth || !P.ray_geo || P.sun_rows < 2 || P.sun_cols < 2) return hipErrorInvalidValue;
+    if (!P.sun_depth || !P.ray_geo || !P.sun_ss || P.sun_rows < 2 || P.sun_cols < 2) return hipErrorInvalidValue;
     if (!sun_table_built)
       hipLaunchKernelGGL(sun_depth_kernel, dim3(blocks_for((long long)P.sun_rows * P.sun_cols)), dim3(kBlock), 0, stream,
                          P);

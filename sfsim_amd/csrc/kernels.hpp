@@ -13,7 +13,7 @@ hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCou
 // Per-(view ray, sample) geometry into P.ray_geo (once per build, before single and ray scatter).
 hipError_t launch_ray_geometry(const DevParams& P, WorkCounters* wc, hipStream_t stream);
 // R1 and M1; with a sun table (P.sun_rows > 0) builds it first (unless sun_table_built: launch_prologue
-// did) and reads P.ray_geo.
+// did) and reads P.ray_geo and the per-build sun map P.sun_ss (launch_resample_map).
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
                                  hipStream_t stream, bool sun_table_built = false);
 // The per-build launches that read nothing of each other, in one launch: direction geometry of the
