@@ -35,8 +35,8 @@ OPS_PER_RAY_SAMPLE = OPS_PER_LOOKUP4 + 6  # ray scatter: one dJ lookup and T(x, 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-seeded", action="store_true", help="skip the surrogate-seeded leg")

@@ -737,48 +737,38 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
 // thread's entry.  The four corners of a cell have distinct parities, so every sample's corners occupy
 // the four slots, and a corner shared with the previous sample is already in its slot: a slot is
 // re-read only when its node changes (about 60 node reads per ray of 64 samples instead of four per
-// sample that changes cell).
+// sample that changes cell).  Which slots a sample re-reads is worked out once per launch (s_mask).
 struct FoldSlots {
   float4 v[4];
-  int key[4];
 };
 
 __device__ __forceinline__ void fold_slots_init(FoldSlots& f) {
-  for (int s = 0; s < 4; ++s) {
-    f.v[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-    f.key[s] = -1;
-  }
-}
-
-__device__ __forceinline__ void fold_slot(const float4* __restrict__ dJ, FoldSlots& f, int s, int o, int kk,
-                                          bool active) {
-  const int key = __builtin_amdgcn_readfirstlane(o);  // -1: no node in this slot (weight zero)
-  if (key >= 0 && key != f.key[s]) {
-    if (active) f.v[s] = dJ[key + kk];
-    f.key[s] = key;
-  }
+  for (int s = 0; s < 4; ++s) f.v[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1):
 // slab[q][kk] = sum over slots of weight x dJ[node][kk].  s_off holds per sample the element offset of
-// each slot's node (-1 when the slot is unused), s_wt their weights (zero for unused slots); the
-// offsets are workgroup-uniform, so are the branches.  With one entry per thread the slots carry over
-// from chunk to chunk.
+// each slot's node (-1 when the slot is unused), s_wt their weights (zero for unused slots), s_mask the
+// slots whose node differs from the previous sample's (bit s); all three are workgroup-uniform, so are
+// the branches.  With one entry per thread the slots carry over from chunk to chunk.
 __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const int4* s_off, const float4* s_wt,
-                                                 float4* slab, int i0, int qn, int NLA, int t, int nt,
-                                                 FoldSlots& f) {
+                                                 const int* s_mask, float4* slab, int i0, int qn, int NLA, int t,
+                                                 int nt, FoldSlots& f) {
   if (NLA <= nt) {
     const int kk = t;
-    const bool active = kk < NLA;
+    const int kl = min(kk, NLA - 1);  // lanes past the table read a valid entry and write nothing
     for (int q = 0; q < qn; ++q) {
-      const int4 o = s_off[i0 + q];
+      const int m = __builtin_amdgcn_readfirstlane(s_mask[i0 + q]);
       const float4 w = s_wt[i0 + q];
-      fold_slot(dJ, f, 0, o.x, kk, active);
-      fold_slot(dJ, f, 1, o.y, kk, active);
-      fold_slot(dJ, f, 2, o.z, kk, active);
-      fold_slot(dJ, f, 3, o.w, kk, active);
+      if (m) {
+        const int4 o = s_off[i0 + q];
+        if (m & 1) f.v[0] = dJ[__builtin_amdgcn_readfirstlane(o.x) + kl];
+        if (m & 2) f.v[1] = dJ[__builtin_amdgcn_readfirstlane(o.y) + kl];
+        if (m & 4) f.v[2] = dJ[__builtin_amdgcn_readfirstlane(o.z) + kl];
+        if (m & 8) f.v[3] = dJ[__builtin_amdgcn_readfirstlane(o.w) + kl];
+      }
       const float4 a = f.v[0], b = f.v[1], c = f.v[2], d = f.v[3];
-      if (active)
+      if (kk < NLA)
         slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
                                          fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
                                          fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
@@ -823,12 +813,16 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
                                                                const float4* __restrict__ geo_rs,
                                                                float4* __restrict__ out, const RsAccum accum, int Q,
                                                                WorkCounters* wc) {
-  __shared__ int4 s_off[kMaxSteps];    // slab offsets (in units of NL x NA) of every sample
-  __shared__ float4 s_wt[kMaxSteps];   // their bilinear weights (h0 e0, h0 e1, h1 e0, h1 e1)
-  extern __shared__ float4 s_slab[];   // 2 x Q x (NL x NA): chunk c in half c % 2
+  // dynamic LDS: 2 x Q x (NL x NA) slab entries (chunk c in half c % 2), then per sample the slab
+  // offsets (in units of NL x NA), their bilinear weights and the fold slots whose node differs from the
+  // previous sample's
+  extern __shared__ float4 s_slab[];
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps;
+  int4* s_off = (int4*)(s_slab + 2 * Q * NLA);
+  float4* s_wt = (float4*)(s_off + n);
+  int* s_mask = (int*)(s_wt + n);
   if ((int)blockIdx.x >= NH * NE) {  // the extra workgroup: E <- tab(dE + E) (atmosphere_lut.clj:95-96)
     const long long nE = (long long)P.nE[0] * P.nE[1];
     for (long long i = threadIdx.x; i < nE; i += blockDim.x) {
@@ -866,16 +860,24 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     s_wt[i] = make_float4(wt(0, 0), wt(0, 1), wt(1, 0), wt(1, 1));
   }
   __syncthreads();
+  // a slot is read when it holds a node that the previous sample's same slot did not (an unused slot
+  // keeps its old value, weighted zero)
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int4 o = s_off[i], p = i > 0 ? s_off[i - 1] : make_int4(-1, -1, -1, -1);
+    s_mask[i] = (o.x >= 0 && o.x != p.x ? 1 : 0) | (o.y >= 0 && o.y != p.y ? 2 : 0) | (o.z >= 0 && o.z != p.z ? 4 : 0) |
+                (o.w >= 0 && o.w != p.w ? 8 : 0);
+  }
+  __syncthreads();
   const int chunks = (n + Q - 1) / Q;
   if (folder) {
     FoldSlots fs;
     fold_slots_init(fs);
-    ray_scatter_fold(dJ, s_off, s_wt, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs);
+    ray_scatter_fold(dJ, s_off, s_wt, s_mask, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs);
     __syncthreads();
     for (int c = 0; c < chunks; ++c) {
       if (c + 1 < chunks)
-        ray_scatter_fold(dJ, s_off, s_wt, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
-                         t, kRSHalf, fs);
+        ray_scatter_fold(dJ, s_off, s_wt, s_mask, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q,
+                         min(Q, n - (c + 1) * Q), NLA, t, kRSHalf, fs);
       __syncthreads();
     }
     return;
@@ -1740,8 +1742,9 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
 }
 
 // Samples per folded-slab chunk of the ray-scatter kernel: its two slab halves take 2 x Q x NL x NA float4
-// of LDS, about 2 x 1024 slab entries (32 KiB, plus 8 KiB of per-sample offsets and weights), so that four
-// 512-thread workgroups fit a CU; SFSIM_ATMO_RAY_CHUNK (1..8, within the same LDS bound) overrides it.
+// of LDS, about 2 x 1024 slab entries (32 KiB, plus 36 B per ray step of offsets, weights and masks), so
+// that four 512-thread workgroups fit a CU; SFSIM_ATMO_RAY_CHUNK (1..8, within the same LDS bound)
+// overrides it.
 int ray_scatter_chunk(const DevParams& P) {
   int nla = P.nS[2] * P.nS[3];
   int q = 1024 / nla;
@@ -1760,7 +1763,8 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   const unsigned rays = (unsigned)(P.nS[0] * P.nS[1]) + (acc.E_out ? 1u : 0u);
   int nla = P.nS[2] * P.nS[3];
   int Q = ray_scatter_chunk(P);
-  size_t shmem = 2 * sizeof(float4) * (size_t)Q * nla;
+  const size_t shmem =
+      2 * sizeof(float4) * (size_t)Q * nla + (size_t)P.ray_steps * (sizeof(int4) + sizeof(float4) + sizeof(int));
   const RayGeoView V = ray_geo_view(P);
   if (nla <= kRSHalf)
     hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc, Q,
