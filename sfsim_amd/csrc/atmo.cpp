@@ -828,24 +828,26 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   TRY(timer.end());
   // the plain iteration's accumulation S_k = tab(S_{k-1} + dS_k), S_0 = R1 (atmosphere_lut.clj:95-97), on
   // the orders dS_k = P_k - P_{k-1}: re-sampling at forward(backward(i)) is not the identity where the
-  // sun direction is clamped, so the order-by-order chain is what the plain tables hold there
+  // sun direction is clamped, so the order-by-order chain is what the plain tables hold there.  Each
+  // step re-samples S + (P_k - P_{k-1}) in one launch; the last one shares its launch with the final
+  // tables of T, E and M1 (atmosphere_lut.clj:98-105)
   const float4* S_cur = f4(ctx->R1);
   int s_next = 0;
   for (int k = 1; k <= kMlpOrders; ++k) {
-    const float4* dk = f4(ctx->pred[0]);
-    if (k > 1) {
-      LAUNCH(timer, kAccumulate, launch_sub4(f4(ctx->pred[k - 1]), f4(ctx->pred[k - 2]), f4(ctx->dJ), n4, ctx->stream));
-      dk = f4(ctx->dJ);
+    ResampleJob jobs[kMaxResampleJobs];
+    int nj = 0;
+    jobs[nj++] = ResampleJob{4, S_cur, f4(ctx->pred[k - 1]), f4(ctx->Sacc[s_next]), nullptr, 0,
+                             k > 1 ? f4(ctx->pred[k - 2]) : nullptr};
+    if (k == kMlpOrders) {
+      jobs[nj++] = ResampleJob{2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, nullptr};
+      jobs[nj++] = ResampleJob{3, f4(ctx->dE[1]), nullptr, nullptr, (float*)ctx->out[1].ptr, 0, nullptr};
+      jobs[nj++] = ResampleJob{4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, nullptr};
     }
-    LAUNCH(timer, kAccumulate, launch_resample(P, 4, S_cur, dk, f4(ctx->Sacc[s_next]), nullptr, 0, ctx->stream));
+    LAUNCH(timer, k == kMlpOrders ? kFinal : kAccumulate, launch_resamples(P, jobs, nj, ctx->stream));
     S_cur = f4(ctx->Sacc[s_next]);
     s_next = 1 - s_next;
   }
-  LAUNCH(timer, kFinal, launch_resample(P, 2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, ctx->stream));
-  LAUNCH(timer, kFinal,
-         launch_resample(P, 3, f4(ctx->dE[1]), nullptr, nullptr, (float*)ctx->out[1].ptr, 0, ctx->stream));
   LAUNCH(timer, kFinal, launch_resample(P, 4, S_cur, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, ctx->stream));
-  LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, ctx->stream));
   return 0;
 }
 

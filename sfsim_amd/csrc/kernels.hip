@@ -266,6 +266,31 @@ __device__ __forceinline__ float3 lookup4(const float4* __restrict__ T, int N1, 
   return mix3(r[0], r[1], a.s);
 }
 
+// lookup4 of the table b - c, each texel's difference formed before the interpolation
+__device__ __forceinline__ float3 lookup4_sub(const float4* __restrict__ b, const float4* __restrict__ c4, int N1, int N2,
+                                              int N3, Lerp a, Lerp bl, Lerp c, Lerp d) {
+  auto ld = [&](long long i) {
+    const float3 u = ld3(b, i), v = ld3(c4, i);
+    return make_float3(u.x - v.x, u.y - v.y, u.z - v.z);
+  };
+  float3 r[2];
+#pragma unroll
+  for (int ia = 0; ia < 2; ++ia) {
+    long long ba = (long long)(ia ? a.i1 : a.i0) * N1;
+    float3 rb[2];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      long long bb = (ba + (ib ? bl.i1 : bl.i0)) * N2;
+      long long c0 = (bb + c.i0) * N3, c1 = (bb + c.i1) * N3;
+      float3 u = mix3(ld(c0 + d.i0), ld(c0 + d.i1), d.s);
+      float3 v = mix3(ld(c1 + d.i0), ld(c1 + d.i1), d.s);
+      rb[ib] = mix3(u, v, c.s);
+    }
+    r[ia] = mix3(rb[0], rb[1], bl.s);
+  }
+  return mix3(r[0], r[1], a.s);
+}
+
 // atmosphere.clj:322-326 in FP32
 __device__ __forceinline__ float sun_elevation_coord(const DevParams& P, int size, float sin_elevation) {
   return (size - 1) * fmaxf(0.0f, (1.0f - __expf(-3.0f * sin_elevation - 0.6f)) * P.sun_inv_den);
@@ -793,7 +818,7 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
 }
 
 __device__ float3 resample4_value(const DevParams& P, const float4* __restrict__ a, const float4* __restrict__ b,
-                                 long long idx);
+                                 long long idx, const float4* __restrict__ c = nullptr);
 __device__ float3 resample4_value_lean(const DevParams& P, const float4* __restrict__ a, long long idx);
 __device__ float3 resample2_value(const DevParams& P, int space, const float4* __restrict__ a,
                                  const float4* __restrict__ b, long long idx);
@@ -1399,10 +1424,11 @@ __global__ void __launch_bounds__(kBlock) prologue_kernel(const DevParams P, con
     resample_map_body(P, b - A.first[6]);
 }
 
-// Value at 4D entry idx of tab(a + b) (b nullable): the multilinear lookup at forward(backward(idx)),
-// from the per-build map when there is one.
+// Value at 4D entry idx of tab(a + b) (b nullable), or of tab(a + (b - c)) with c (each texel's
+// difference formed first): the multilinear lookup at forward(backward(idx)), from the per-build map
+// when there is one.
 __device__ float3 resample4_value(const DevParams& P, const float4* __restrict__ a, const float4* __restrict__ b,
-                                 long long idx) {
+                                 long long idx, const float4* __restrict__ c) {
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const long long n = (long long)NH * NE * NL * NA;
   Lerp Lh, Le, Ls, La;
@@ -1423,16 +1449,20 @@ __device__ float3 resample4_value(const DevParams& P, const float4* __restrict__
   float3 v;
   if (Lh.s == 0.0f && Le.s == 0.0f && Ls.s == 0.0f && La.s == 0.0f) {
     // the entry maps onto a grid point (every weight zero): the lookup is that point's value, bit for bit
-    const long long c = (((long long)Lh.i0 * NE + Le.i0) * NL + Ls.i0) * NA + La.i0;
-    v = ld3(a, c);
+    const long long g = (((long long)Lh.i0 * NE + Le.i0) * NL + Ls.i0) * NA + La.i0;
+    v = ld3(a, g);
     if (b) {
-      const float3 w = ld3(b, c);
+      float3 w = ld3(b, g);
+      if (c) {
+        const float3 m = ld3(c, g);
+        w = make_float3(w.x - m.x, w.y - m.y, w.z - m.z);
+      }
       v.x += w.x; v.y += w.y; v.z += w.z;
     }
   } else {
     v = lookup4(a, NE, NL, NA, Lh, Le, Ls, La);
     if (b) {
-      float3 w = lookup4(b, NE, NL, NA, Lh, Le, Ls, La);
+      const float3 w = c ? lookup4_sub(b, c, NE, NL, NA, Lh, Le, Ls, La) : lookup4(b, NE, NL, NA, Lh, Le, Ls, La);
       v.x += w.x; v.y += w.y; v.z += w.z;
     }
   }
@@ -1498,13 +1528,14 @@ __device__ float3 resample2_value(const DevParams& P, int space, const float4* _
 
 __device__ __forceinline__ void resample_entry(const DevParams& P, int space, const float4* __restrict__ a,
                                                const float4* __restrict__ b, float4* __restrict__ out4,
-                                               float* __restrict__ out_rgb, int tiled, long long idx) {
+                                               float* __restrict__ out_rgb, int tiled, long long idx,
+                                               const float4* __restrict__ c = nullptr) {
   float3 v;
   long long dst = idx;
   if (space == 4) {
     const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
     if (idx >= (long long)NH * NE * NL * NA) return;
-    v = resample4_value(P, a, b, idx);
+    v = resample4_value(P, a, b, idx, c);
     if (tiled) {  // image.clj:299-312: row h * NL + s, column e * NA + a
       const int ia = (int)(idx % NA);
       long long q = idx / NA;
@@ -1544,7 +1575,7 @@ __global__ void __launch_bounds__(kBlock) resample_jobs_kernel(const DevParams P
   while (k + 1 < J.count && blockIdx.x >= J.first[k + 1]) ++k;
   const ResampleJob& j = J.job[k];
   resample_entry(P, j.space, j.a, j.b, j.out4, j.out_rgb, j.tiled,
-                 (long long)(blockIdx.x - J.first[k]) * blockDim.x + threadIdx.x);
+                 (long long)(blockIdx.x - J.first[k]) * blockDim.x + threadIdx.x, j.c);
 }
 
 __global__ void pack_rgb_kernel(const float4* __restrict__ in, float* __restrict__ out, long long count) {
@@ -1568,14 +1599,6 @@ __global__ void add4_kernel(const float4* __restrict__ a, const float4* __restri
   if (i >= count) return;
   float4 x = a[i], y = b[i];
   c[i] = make_float4(x.x + y.x, x.y + y.y, x.z + y.z, 0.f);
-}
-
-__global__ void sub4_kernel(const float4* __restrict__ a, const float4* __restrict__ b, float4* __restrict__ c,
-                            long long count) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  float4 x = a[i], y = b[i];
-  c[i] = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, 0.f);
 }
 
 // Block maxima of |a + x| per channel: partial[block] (x, y, z).  Two passes without atomics (the
@@ -1843,7 +1866,8 @@ hipError_t launch_resamples(const DevParams& P, const ResampleJob* jobs, int cou
   for (int k = 0; k < kMaxResampleJobs; ++k) {
     J.job[k] = jobs[k < count ? k : 0];
     if (k < count) {
-      if (jobs[k].space < 2 || jobs[k].space > 4 || !jobs[k].a) return hipErrorInvalidValue;
+      if (jobs[k].space < 2 || jobs[k].space > 4 || !jobs[k].a || (jobs[k].c && (jobs[k].space != 4 || !jobs[k].b)))
+        return hipErrorInvalidValue;
       J.first[k + 1] = J.first[k] + blocks_for(resample_count(P, jobs[k].space));
     }
   }
@@ -1856,11 +1880,6 @@ hipError_t launch_resample(const DevParams& P, int space, const float4* a, const
   const long long n = resample_count(P, space);
   hipLaunchKernelGGL(resample_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, space, a, b, out4, out_rgb,
                      tiled);
-  return hipGetLastError();
-}
-
-hipError_t launch_sub4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream) {
-  hipLaunchKernelGGL(sub4_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, stream, a, b, c, count);
   return hipGetLastError();
 }
 

@@ -82,7 +82,7 @@ hipError_t launch_resample_map(const DevParams& P, hipStream_t stream);
 hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
                            float* out_rgb, int tiled, hipStream_t stream);
 // Up to kMaxResampleJobs independent launch_resample jobs in one launch (no job may write what another
-// reads).
+// reads).  c (space 4, with b): the job re-samples a + (b - c), each texel's difference formed first.
 struct ResampleJob {
   int space;
   const float4* a;
@@ -90,13 +90,12 @@ struct ResampleJob {
   float4* out4;
   float* out_rgb;
   int tiled;
+  const float4* c;
 };
 constexpr int kMaxResampleJobs = 4;
 hipError_t launch_resamples(const DevParams& P, const ResampleJob* jobs, int count, hipStream_t stream);
 // c = a + b (float4 tables)
 hipError_t launch_add4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream);
-// c = a - b (RGB, w = 0)
-hipError_t launch_sub4(const float4* a, const float4* b, float4* c, long long count, hipStream_t stream);
 // scratch[4] <- max_i max_c |x - y| / max(|a + x|, fraction * max_i |a + x|_c) (scratch[0..2]: the
 // channel maxima); scratch holds residual_scratch_floats(count) floats.  Block reductions in two passes,
 // no atomics, no memset.
