@@ -145,8 +145,9 @@ def seeded_leg(atm, ctx, P, workload, steps, train_steps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the LUT precompute on a single MI355X "
-              f"(--gpus {args.gpus} is not supported; see DESIGN.md for how the work would shard).", file=sys.stderr)
+        print(f"bench.py measures one GPU: the LUT precompute runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; across GPUs the work splits by atmosphere: independent "
+              f"builds, one process per GPU, no exchange between stages).", file=sys.stderr)
         sys.exit(2)
     from sfsim_amd import atmosphere as atm
     P = atm.workload(args.workload)
