@@ -12,7 +12,7 @@ Prints one JSON line with the throughput (LUT entries/s), the roofline of the do
 (ray scattering, five launches per build, measured with HIP events on the library's stream) and the
 FP64 CPU oracle timed on a bounded sample of the same workload on this host.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 10] [--no-cpu-baseline]
 """
 import argparse
 import json
