@@ -117,6 +117,24 @@ def test_stage_ray_scatter_small(gpu, oracle, small_inputs):
     assert_close("dS", gpu.ray_scatter_table(P, J1), oracle.stage_ray_scatter(OP, J1), TOL_SCATTER)
 
 
+def test_stage_calls_between_builds_use_their_own_maps(atm, gpu, oracle, small_inputs):
+    """Every build and every single-stage call writes the per-build re-sampling and sun maps for its own
+    grid before the kernels that read them: stage calls on the small grid between two builds of the full
+    workload match the oracle, and the second build (a graph replay) equals the first bit for bit."""
+    P, OP, R1, M1, E0, J1 = small_inputs
+    F = atm.workload("full-5-order")
+    gpu.build(F)
+    first = [gpu.fetch(F, i) for i in range(4)]
+    assert_close("dJ1", gpu.point_scatter_table(P, R1, E0, mie_strength=M1), J1, TOL_SCATTER)
+    assert_close("dS", gpu.ray_scatter_table(P, J1), oracle.stage_ray_scatter(OP, J1), TOL_SCATTER)
+    R1g, M1g = gpu.single_scatter_tables(P)
+    assert_close("R1", R1g, R1, TOL_SCATTER)
+    assert_close("M1", M1g, M1, TOL_SCATTER)
+    gpu.build(F)
+    for i in range(4):
+        assert np.array_equal(gpu.fetch(F, i), first[i]), i
+
+
 def test_stage_surface_radiance_small(gpu, oracle, small_inputs):
     P, OP, R1, M1, E0, J1 = small_inputs
     assert_close("dE1", gpu.surface_radiance_table(P, R1, mie_strength=M1),
