@@ -1092,16 +1092,20 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
                                                      float row_bytes, float lx, float ly, float lz) {
   float3 acc = make_float3(0.f, 0.f, 0.f);
   const unsigned d_row = (unsigned)(Ls.i1 - Ls.i0) * (unsigned)NA * (unsigned)sizeof(float4);
-#pragma unroll 2
-  for (int j = 0; j < ndirs; ++j) {
+  // direction j's heading coordinate: cosine ol, column weight sa, byte offset of the lower column
+  auto heading = [&](int j, float& ol, float& sa) {
     const DirLds& dd = s_dir[j];
-    const float ol = fmaf(dd.omega.z, lz, fmaf(dd.omega.y, ly, dd.omega.x * lx));
+    ol = fmaf(dd.omega.z, lz, fmaf(dd.omega.y, ly, dd.omega.x * lx));
     const float hc = __builtin_amdgcn_fmed3f(fmaf(pc.hA, ol, pc.hA), 0.0f, pc.na_top);
     const float col = lower_cell(hc, pc.na_top - 1.0f);
-    const float sa = hc - col;
-    const unsigned off = (unsigned)fmaf(col, (float)sizeof(float4), row_bytes);
+    sa = hc - col;
+    return (unsigned)fmaf(col, (float)sizeof(float4), row_bytes);
+  };
+  // the rest of direction j's term given the two texels c0, c1 of its lower slab row
+  auto term = [&](int j, float ol, float sa, unsigned off, float4 c0, float4 c1) {
+    const DirLds& dd = s_dir[j];
     const float4* row = sh + j * NLA;
-    float3 v = lerp3(ld4b(row, off), ld4b(row, off + 16u), sa);
+    float3 v = lerp3(c0, c1, sa);
     if (!kRow) {
       const float3 u = lerp3(ld4b(row, off + d_row), ld4b(row, off + d_row + 16u), sa);
       v = make_float3(fmaf(Ls.s, u.x - v.x, v.x), fmaf(Ls.s, u.y - v.y, v.y), fmaf(Ls.s, u.z - v.z, v.z));
@@ -1133,6 +1137,29 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
     acc.x = fmaf(dd.weight.x, v.x, acc.x);
     acc.y = fmaf(dd.weight.y, v.y, acc.y);
     acc.z = fmaf(dd.weight.z, v.z, acc.z);
+  };
+  // four directions at a time: their eight slab reads are issued before any of their ground terms
+  constexpr int kB = 4;
+  int j = 0;
+  for (; j + kB <= ndirs; j += kB) {
+    float ol[kB], sa[kB];
+    unsigned off[kB];
+    float4 c0[kB], c1[kB];
+#pragma unroll
+    for (int d = 0; d < kB; ++d) {
+      off[d] = heading(j + d, ol[d], sa[d]);
+      const float4* row = sh + (j + d) * NLA;
+      c0[d] = ld4b(row, off[d]);
+      c1[d] = ld4b(row, off[d] + 16u);
+    }
+#pragma unroll
+    for (int d = 0; d < kB; ++d) term(j + d, ol[d], sa[d], off[d], c0[d], c1[d]);
+  }
+  for (; j < ndirs; ++j) {
+    float ol, sa;
+    const unsigned off = heading(j, ol, sa);
+    const float4* row = sh + j * NLA;
+    term(j, ol, sa, off, ld4b(row, off), ld4b(row, off + 16u));
   }
   return acc;
 }
