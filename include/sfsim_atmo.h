@@ -77,8 +77,9 @@ const char* sfsim_atmo_last_error(const sfsim_atmo_ctx* ctx);
 /* How the single-scatter stage gets the optical depth of each sample's sun segment (the transmittance
  * toward the sun of atmosphere.clj:154-160).  rows x cols in [16, 8192]: from a table of the segments'
  * midpoint sums over (horizon distance, exit distance), built at the start of each build and read with
- * bilinear interpolation in log2; with 768 x 768 (the default) the final tables of the 5-order workload
- * stay within 5.4e-5 of the per-segment sums (1024 x 1024: 4.6e-5, 6% slower).  0 x 0: every segment
+ * bilinear interpolation in log2; with 640 x 768 (the default, 3.75 MiB) the final tables of the 5-order
+ * workload stay within 5.8e-5 of the FP64 oracle (768 x 768: 5.4e-5, slightly slower; 1024 x 1024:
+ * 4.6e-5, 6% slower).  0 x 0: every segment
  * summed node by node, as the reference does (about 7x the single-scatter time, optical depths exact to
  * FP32 rounding).  No counterpart in the
  * reference, which always sums per segment.  Takes effect at the next build. */

@@ -47,11 +47,11 @@ struct DeviceBuffer {
 
 }  // namespace
 
-// Default size of the sun optical-depth table: 768 horizon-distance rows x 768 exit-distance columns
-// (4.5 MiB, within one XCD's 4 MiB L2 for the rows that a height range reads): the final tables stay
-// within 5.4e-5 of the per-segment sums at the full workload (4.6e-5 at 1024 x 1024, which takes 6% longer
-// per build because its gathers miss the L2).
-constexpr int kSunRows = 768, kSunCols = 768;
+// Default size of the sun optical-depth table: 640 horizon-distance rows x 768 exit-distance columns
+// (3.75 MiB, within one XCD's 4 MiB L2): the final tables stay within 5.8e-5 of the FP64 oracle at the
+// full workload (768 x 768: 5.4e-5 and ~5 us slower per build, its 4.5 MiB refetched beyond the L2;
+// 1024 x 1024: 4.6e-5, 6% slower; fewer columns cost more accuracy than fewer rows: 768 x 640 9.8e-5).
+constexpr int kSunRows = 640, kSunCols = 768;
 
 struct sfsim_atmo_ctx {
   int device = 0;
