@@ -37,7 +37,8 @@ hipError_t launch_direction_geometry(const DevParams& P, const Dir* dirs, int nd
 size_t direction_geometry_bytes(const DevParams& P, int ndirs);
 // dS: previous-order scatter table; mie (nullable): first-iteration Mie strength table, added with the
 // Mie phase of the quadrature direction; dE: previous-order surface-radiance table.  slab / slab_mie
-// are scratch tables of NH x ndirs x NL x NA float4 (slab_mie used only when mie is given).
+// are scratch tables of NH x ndirs x NL x NA float4 (slab_mie used only when mie is given).  Reads the
+// per-build sun map P.sun_ps (launch_resample_map or launch_prologue).
 // The surface radiance of the same dS and Mie strength (launch_surface_radiance) computed in the
 // point-scatter launch: ndirs half-sphere directions, output table out; count is set by the launcher.
 struct SrJob {
@@ -65,7 +66,8 @@ struct RsAccum {
   const float4* E_in = nullptr;
   float4* E_out = nullptr;
 };
-// Reads P.ray_geo (launch_ray_geometry).
+// Reads P.ray_geo (launch_ray_geometry) and the per-build sun map P.sun_rs (launch_resample_map or
+// launch_prologue).
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream, const RsAccum& acc = RsAccum());
 // Half-sphere direction geometry of every surface-radiance height into P.sgeo (once per build, before
