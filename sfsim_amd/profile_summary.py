@@ -6,6 +6,8 @@
         <calib_mib> <kernel-regex> <out.json> <command...>
 """
 import csv
+import os
+import statistics
 import json
 import re
 import sys
@@ -17,14 +19,26 @@ def _short(name):
 
 
 def stats(path, out, command):
+    """Kernel statistics table; with the kernel trace next to the stats file (k_kernel_trace.csv for
+    k_kernel_stats.csv) also each kernel's median duration, which the few cold first launches do not move."""
     rows = list(csv.DictReader(open(path)))
     total = sum(float(r["TotalDurationNs"]) for r in rows)
-    lines = [f"# rocprofv3 kernel trace: {command}", "", "| kernel | calls | total ms | average us | min us | max us | % |",
-             "|---|---|---|---|---|---|---|"]
+    medians = {}
+    trace = path.replace("_kernel_stats.csv", "_kernel_trace.csv")
+    if trace != path and os.path.exists(trace):
+        durations = {}
+        for t in csv.DictReader(open(trace)):
+            durations.setdefault(t["Kernel_Name"], []).append(int(t["End_Timestamp"]) - int(t["Start_Timestamp"]))
+        medians = {k: statistics.median(v) for k, v in durations.items()}
+    lines = [f"# rocprofv3 kernel trace: {command}", "",
+             "| kernel | calls | total ms | average us | median us | min us | max us | % |",
+             "|---|---|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        med = medians.get(r["Name"])
+        med_s = f"{med / 1e3:.1f}" if med is not None else "-"
         lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
-                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
-                     f"{100 * float(r['TotalDurationNs']) / total:.1f} |")
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {med_s} | {float(r['MinNs']) / 1e3:.1f} | "
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {100 * float(r['TotalDurationNs']) / total:.1f} |")
     open(out, "w").write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
