@@ -29,10 +29,14 @@
  *     scattering coefficient, the optically thin limit;
  *     tests/test_oracle_invariants.py).
  *   - order 2 of the driver: the same brute-force evaluator carried one order
- *     further (tests/test_oracle_bruteforce.py).  Orders >= 3: parity
- *     unpinned by reference-held vectors; they repeat the pinned stages, with
- *     the bookkeeping rebuilt from the stage functions in
- *     tests/test_oracle_invariants.py.
+ *     further (tests/test_oracle_bruteforce.py).
+ *   - the multi-order driver (orc_generate_atmosphere_luts): the structure the
+ *     truncated series must have when the scattering coefficients (extinction
+ *     fixed) and the albedo are scaled (tests/test_oracle_driver_pin.py): the
+ *     first iteration adds exactly the brute-force-pinned second order, every
+ *     further one a single monomial of the next degree, and the albedo enters
+ *     with the degree the ground reflections fix.  Mis-wired drivers (a stage
+ *     reading the wrong order's table, E0 kept in E, S accumulating dJ) fail it.
  */
 #include <math.h>
 #include <stdint.h>
@@ -927,9 +931,8 @@ static size_t count_of(const int* shape, int ndim) {
 }
 
 /* atmosphere_lut.clj:43-105: the whole driver.  Outputs are the final make-lookup-table results
- * (grid order, 3 doubles per entry, before the 4D->2D tiling of the file writes).
- * The optional stage_out receives, when non-NULL, the per-iteration tables in this order:
- * T, E0, R1, M1, then per iteration dJ, dE, dS. */
+ * (grid order, 3 doubles per entry, before the 4D->2D tiling of the file writes).  threads and
+ * structure as for the stages (structure == 1: every stage parallel over dimension 0 only). */
 int orc_generate_atmosphere_luts(const orc_params* P, double* T_out, double* E_out, double* S_out, double* M_out,
                                  int threads, int structure) {
   size_t nT = count_of(P->transmittance_shape, 2), nE = count_of(P->surface_radiance_shape, 2);

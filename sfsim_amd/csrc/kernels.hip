@@ -1167,7 +1167,8 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
 // kELds: the previous order's dE at the ground (height row 0) is staged in LDS after the direction
 // table (when it fits 16 KiB).  P, the ground point of a surface direction, is the ray's intersection
 // with the planet's sphere (atmosphere.clj:80-85, 213-221), so its height is 0: its FP64 |P| - R is
-// rounding noise of ~1e-9 m, whose height coordinate (~1e-13) would weight height row 1 by as much.
+// rounding noise of ~1e-9 m, whose height coordinate (through the square root of the horizon distance,
+// ~1.5e-6 for 16 rows and 100 km) would weight height row 1 by as much.
 template <bool kFold>
 __device__ __forceinline__ void surface_radiance_entry(const DevParams& P, int ndirs, const float4* __restrict__ dS,
                                                        const float4* __restrict__ mie, float4* __restrict__ out,
