@@ -52,7 +52,9 @@ typedef struct sfsim_atmo_scatter {
   double scatter_g;
 } sfsim_atmo_scatter;
 
-/* Everything generate-atmosphere-luts hard-codes (atmosphere_lut.clj:20-62). */
+/* Everything generate-atmosphere-luts hard-codes (atmosphere_lut.clj:20-62).  Every field sits at its
+ * natural alignment and the size (232 bytes) is a multiple of 8, so a packed description of the fields in
+ * this order (coffi's [::mem/struct ...], ctypes) has exactly this layout, in arrays too. */
 typedef struct sfsim_atmo_params {
   sfsim_atmo_planet planet;
   sfsim_atmo_scatter mie;      /* scatter = [mie rayleigh] (atmosphere_lut.clj:63) */
@@ -64,6 +66,7 @@ typedef struct sfsim_atmo_params {
   int ray_steps;                 /* integral-ray steps; also the half-sphere steps of surface-radiance */
   int sphere_steps;              /* integral-sphere steps of point-scatter */
   int iterations;                /* scattering-order iterations */
+  int reserved;                  /* set to 0; makes the 4-byte tail explicit: 232 bytes, no padding anywhere */
 } sfsim_atmo_params;
 
 typedef struct sfsim_atmo_ctx sfsim_atmo_ctx;

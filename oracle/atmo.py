@@ -78,7 +78,7 @@ def lib():
         _lib.orc_stage_point_scatter.argtypes = [PP, i32, _dp, _dp, _dp, _lp, i64, _dp, i32, i32]
         _lib.orc_stage_ray_scatter.argtypes = [PP, _dp, _lp, i64, _dp, i32, i32]
         _lib.orc_stage_surface_radiance.argtypes = [PP, i32, _dp, _dp, _lp, i64, _dp, i32, i32]
-        _lib.orc_stage_resample.argtypes = [PP, i32, _dp, _dp, _lp, i64, _dp, i32]
+        _lib.orc_stage_resample.argtypes = [PP, i32, _dp, _dp, _lp, i64, _dp, i32, i32]
         _lib.orc_generate_atmosphere_luts.argtypes = [PP, _dp, _dp, _dp, _dp, i32, i32]
     return _lib
 
@@ -450,13 +450,13 @@ def stage_surface_radiance(P, first, dS_A, dS_B, indices=None, threads=0, struct
     return out
 
 
-def stage_resample(P, space, A, B=None, indices=None, threads=0):
+def stage_resample(P, space, A, B=None, indices=None, threads=0, structure=0):
     shape = {2: P.transmittance_shape, 3: P.surface_radiance_shape, 4: P.scatter_shape}[space]
     arr, n = _idx(indices)
     out = _out_for(shape, n)
     a, pa = _tab(A)
     b, pb = _tab(B)
-    lib().orc_stage_resample(ctypes.byref(P), space, pa, pb, _lptr(arr), n, _ptr(out), threads)
+    lib().orc_stage_resample(ctypes.byref(P), space, pa, pb, _lptr(arr), n, _ptr(out), threads, structure)
     return out
 
 
@@ -509,6 +509,75 @@ def glsl_surface_radiance(p, shape, E, point, light):
     e, pe = _tab(E)
     out = _out3()
     lib().orc_glsl_surface_radiance(ctypes.byref(p), shp, pe, _ptr(_vec(point)), _ptr(_vec(light)), _ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- GLSL texture path over tiled tables
+_ci = ctypes.c_int
+
+
+def glsl_convert_2d_index(idx, size_y, size_x):
+    out = np.zeros(2)
+    f = lib().orc_glsl_convert_2d_index
+    f.argtypes = [_dp, _ci, _ci, _dp]
+    f(_ptr(_vec(idx)), size_y, size_x, _ptr(out))
+    return out
+
+
+def glsl_make_2d_index_from_4d(idx, size_w, size_z, size_y, size_x):
+    out = np.zeros(4)
+    f = lib().orc_glsl_make_2d_index_from_4d
+    f.argtypes = [_dp, _ci, _ci, _ci, _ci, _dp]
+    f(_ptr(_vec(idx)), size_w, size_z, size_y, size_x, _ptr(out))
+    return out
+
+
+def glsl_interpolate_2d(tex, size_y, size_x, idx):
+    """interpolate-2d.glsl on a texture of size_y rows and size_x columns (RGB per texel, row-major)."""
+    t, pt = _tab(tex)
+    out = _out3()
+    f = lib().orc_glsl_interpolate_2d
+    f.argtypes = [_dp, _ci, _ci, _dp, _dp]
+    f(pt, size_y, size_x, _ptr(_vec(idx)), _ptr(out))
+    return out
+
+
+def glsl_interpolate_4d(tex, size_w, size_z, size_y, size_x, idx):
+    """interpolate-4d.glsl on the tiled texture (size_y size_w rows, size_x size_z columns)."""
+    t, pt = _tab(tex)
+    out = _out3()
+    f = lib().orc_glsl_interpolate_4d
+    f.argtypes = [_dp, _ci, _ci, _ci, _ci, _dp, _dp]
+    f(pt, size_w, size_z, size_y, size_x, _ptr(_vec(idx)), _ptr(out))
+    return out
+
+
+def glsl_tiled_transmittance_outer(p, shape, T, point, direction):
+    t, pt = _tab(T)
+    out = _out3()
+    f = lib().orc_glsl_tiled_transmittance_outer
+    f.argtypes = [ctypes.POINTER(Planet), ctypes.POINTER(_ci), _dp, _dp, _dp, _dp]
+    f(ctypes.byref(p), (_ci * 2)(*shape), pt, _ptr(_vec(point)), _ptr(_vec(direction)), _ptr(out))
+    return out
+
+
+def glsl_tiled_ray_scatter_outer(p, shape, S, M, g, light, point, direction):
+    s, ps = _tab(S)
+    m, pm = _tab(M)
+    out = _out3()
+    f = lib().orc_glsl_tiled_ray_scatter_outer
+    f.argtypes = [ctypes.POINTER(Planet), ctypes.POINTER(_ci), _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+    f(ctypes.byref(p), (_ci * 4)(*shape), ps, pm, g, _ptr(_vec(light)), _ptr(_vec(point)), _ptr(_vec(direction)),
+      _ptr(out))
+    return out
+
+
+def glsl_tiled_surface_radiance(p, shape, E, point, light):
+    e, pe = _tab(E)
+    out = _out3()
+    f = lib().orc_glsl_tiled_surface_radiance
+    f.argtypes = [ctypes.POINTER(Planet), ctypes.POINTER(_ci), _dp, _dp, _dp, _dp]
+    f(ctypes.byref(p), (_ci * 2)(*shape), pe, _ptr(_vec(point)), _ptr(_vec(light)), _ptr(out))
     return out
 
 

@@ -718,7 +718,8 @@ void orc_convert_4d_to_2d(const double* in, const int* shape, double* out) {
 /* Each stage evaluates f(backward(i)) (interpolate.clj:54-72) for the flat grid indices given
  * (or for every grid point when indices == NULL).  Threads: 0 -> all entries in parallel;
  * otherwise that many OpenMP threads.  structure == 1 parallelises over dimension 0 only, as
- * pmap over the first table dimension does in interpolate.clj:59,72. */
+ * pmap over the first table dimension does in interpolate.clj:59,72 (given indices must then be
+ * sorted ascending). */
 
 typedef struct {
   const orc_params* P;
@@ -749,6 +750,26 @@ static void run_stage(const build_ctx* b, const void* stage, entry_fn f, const i
 #else
   int nthreads = 1; (void)threads;
 #endif
+  if (indices != NULL && structure) {
+    /* a sample of the reference structure: the given entries (sorted ascending) grouped by their index
+     * along dimension 0, one group per task, at most shape[0] tasks at a time */
+    size_t inner = total / (size_t)shape[0];
+    int nt = nthreads < shape[0] ? nthreads : shape[0];
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int i0 = 0; i0 < shape[0]; i0++) {
+      int64_t lo = 0, hi = count;
+      while (lo < hi) { /* first entry with flat index >= i0 * inner */
+        int64_t mid = lo + (hi - lo) / 2;
+        if ((size_t)indices[mid] < (size_t)i0 * inner) lo = mid + 1; else hi = mid;
+      }
+      for (int64_t k = lo; k < count && (size_t)indices[k] < (size_t)(i0 + 1) * inner; k++) {
+        double idx[4];
+        unflatten((size_t)indices[k], shape, ndim, idx);
+        f(b, stage, idx, out + (size_t)k * 3);
+      }
+    }
+    return;
+  }
   if (indices == NULL && structure) {
     size_t inner = total / (size_t)shape[0];
     int nt = nthreads < shape[0] ? nthreads : shape[0];
@@ -917,11 +938,11 @@ void orc_stage_surface_radiance(const orc_params* P, int first, const double* dS
   run_stage(&b, &s, entry_surface_radiance, P->surface_radiance_shape, 2, indices, count, out, threads, structure);
 }
 void orc_stage_resample(const orc_params* P, int space, const double* A, const double* B, const int64_t* indices,
-                        int64_t count, double* out, int threads) {
+                        int64_t count, double* out, int threads, int structure) {
   build_ctx b; make_ctx(&b, P);
   const int* shape = space == 2 ? P->transmittance_shape : space == 3 ? P->surface_radiance_shape : P->scatter_shape;
   resample_stage s = {space, shape, A, B};
-  run_stage(&b, &s, entry_resample, shape, space == 4 ? 4 : 2, indices, count, out, threads, 0);
+  run_stage(&b, &s, entry_resample, shape, space == 4 ? 4 : 2, indices, count, out, threads, structure);
 }
 
 static size_t count_of(const int* shape, int ndim) {
@@ -969,18 +990,18 @@ int orc_generate_atmosphere_luts(const orc_params* P, double* T_out, double* E_o
     { double* t = dS; dS = dS_new; dS_new = t; }
     first = 0;
     /* E <- table(E + dE); the constant-zero E contributes exactly 0 */
-    orc_stage_resample(P, 3, dE, e_is_zero ? NULL : E, NULL, 0, E_new, threads);
+    orc_stage_resample(P, 3, dE, e_is_zero ? NULL : E, NULL, 0, E_new, threads, structure);
     { double* t = E; E = E_new; E_new = t; }
     e_is_zero = 0;
     /* S <- table(S + dS) */
-    orc_stage_resample(P, 4, S, dS, NULL, 0, S_new, threads);
+    orc_stage_resample(P, 4, S, dS, NULL, 0, S_new, threads, structure);
     { double* t = S; S = S_new; S_new = t; }
   }
-  orc_stage_resample(P, 2, T, NULL, NULL, 0, T_out, threads);
+  orc_stage_resample(P, 2, T, NULL, NULL, 0, T_out, threads, structure);
   if (e_is_zero) memset(E_out, 0, nE * 3 * sizeof(double));
-  else orc_stage_resample(P, 3, E, NULL, NULL, 0, E_out, threads);
-  orc_stage_resample(P, 4, S, NULL, NULL, 0, S_out, threads);
-  orc_stage_resample(P, 4, M1, NULL, NULL, 0, M_out, threads);
+  else orc_stage_resample(P, 3, E, NULL, NULL, 0, E_out, threads, structure);
+  orc_stage_resample(P, 4, S, NULL, NULL, 0, S_out, threads, structure);
+  orc_stage_resample(P, 4, M1, NULL, NULL, 0, M_out, threads, structure);
   free(T); free(dE); free(dE_new); free(E); free(E_new); free(R1); free(M1); free(dJ); free(dS); free(dS_new);
   free(S); free(S_new);
   return 0;
@@ -1044,6 +1065,115 @@ void orc_glsl_surface_radiance(const orc_planet* p, const int* shape, const doub
   double idx[2];
   surface_radiance_forward(p, shape, arr3(point), arr3(light), idx);
   store(out, interpolate_value(E, shape, 2, idx));
+}
+
+/* ---------------------- GLSL texture path over the tiled .scatter tables (FP64) */
+/* The renderer uploads each file as one 2D RGB texture with GL_LINEAR filtering and CLAMP_TO_EDGE
+ * (atmosphere.clj:587-619; texture.clj:238-245, 380-398: the 4D tables as a (heading x elevation) wide,
+ * (sun elevation x height) high image) and reads it through these shader functions.  texture() is
+ * restated with exact bilinear weights (the hardware's are fixed point, hence the reference tests'
+ * 1e-4 tolerances on GL results). */
+
+/* texture(sampler2D, vec2(s, t)) of a width x height texture, texel (i, j) at tex[(j width + i) 3] */
+static v3 gl_texture(const double* tex, int width, int height, double s, double t) {
+  double u = s * width - 0.5, v = t * height - 0.5;
+  double fu = floor(u), fv = floor(v);
+  double a = u - fu, b = v - fv;
+  int i0 = (int)fu, j0 = (int)fv, i1 = i0 + 1, j1 = j0 + 1;
+  i0 = i0 < 0 ? 0 : (i0 > width - 1 ? width - 1 : i0);
+  i1 = i1 < 0 ? 0 : (i1 > width - 1 ? width - 1 : i1);
+  j0 = j0 < 0 ? 0 : (j0 > height - 1 ? height - 1 : j0);
+  j1 = j1 < 0 ? 0 : (j1 > height - 1 ? height - 1 : j1);
+  v3 t00 = arr3(tex + ((size_t)j0 * width + i0) * 3), t10 = arr3(tex + ((size_t)j0 * width + i1) * 3);
+  v3 t01 = arr3(tex + ((size_t)j1 * width + i0) * 3), t11 = arr3(tex + ((size_t)j1 * width + i1) * 3);
+  v3 lo = vadd(vmul(t00, 1 - a), vmul(t10, a)), hi = vadd(vmul(t01, 1 - a), vmul(t11, a));
+  return vadd(vmul(lo, 1 - b), vmul(hi, b));
+}
+
+/* core/convert-2d-index.glsl: normalised index (x, y) -> texture coordinates of texel centres */
+void orc_glsl_convert_2d_index(const double* idx, int size_y, int size_x, double* out) {
+  out[0] = (idx[0] * (size_x - 1) + 0.5) / size_x;
+  out[1] = (idx[1] * (size_y - 1) + 0.5) / size_y;
+}
+
+/* core/interpolate-2d.glsl */
+static v3 glsl_interpolate_2d(const double* tex, int size_y, int size_x, const double* idx) {
+  double c[2];
+  orc_glsl_convert_2d_index(idx, size_y, size_x, c);
+  return gl_texture(tex, size_x, size_y, c[0], c[1]);
+}
+void orc_glsl_interpolate_2d(const double* tex, int size_y, int size_x, const double* idx, double* out) {
+  store(out, glsl_interpolate_2d(tex, size_y, size_x, idx));
+}
+
+/* core/make-2d-index-from-4d.glsl: pixel index (x heading, y sun elevation, z elevation, w height) ->
+ * (s, t, p, q): texture columns of elevation tiles floor(z) and floor(z) + 1 (clamped), rows of height
+ * tiles floor(w) and floor(w) + 1 (clamped) */
+void orc_glsl_make_2d_index_from_4d(const double* idx, int size_w, int size_z, int size_y, int size_x, double* out) {
+  double z_floor = floor(idx[2]), w_floor = floor(idx[3]);
+  double dx = (double)size_x * size_z, dy = (double)size_y * size_w;
+  out[0] = (0.5 + idx[0] + z_floor * size_x) / dx;
+  out[1] = (0.5 + idx[0] + fmin(z_floor + 1, size_z - 1) * size_x) / dx;
+  out[2] = (0.5 + idx[1] + w_floor * size_y) / dy;
+  out[3] = (0.5 + idx[1] + fmin(w_floor + 1, size_w - 1) * size_y) / dy;
+}
+
+/* core/interpolate-4d.glsl: heading and sun elevation filtered by the texture unit, elevation and height
+ * mixed explicitly */
+static v3 glsl_interpolate_4d(const double* tex, int size_w, int size_z, int size_y, int size_x, const double* idx) {
+  double pixel[4] = {idx[0] * (size_x - 1), idx[1] * (size_y - 1), idx[2] * (size_z - 1), idx[3] * (size_w - 1)};
+  double fs = pixel[2] - floor(pixel[2]), ft = pixel[3] - floor(pixel[3]);
+  double c[4];
+  orc_glsl_make_2d_index_from_4d(pixel, size_w, size_z, size_y, size_x, c);
+  int width = size_x * size_z, height = size_y * size_w;
+  v3 sp = gl_texture(tex, width, height, c[0], c[2]), tp = gl_texture(tex, width, height, c[1], c[2]);
+  v3 sq = gl_texture(tex, width, height, c[0], c[3]), tq = gl_texture(tex, width, height, c[1], c[3]);
+  return vadd(vadd(vmul(sp, (1 - fs) * (1 - ft)), vmul(tp, fs * (1 - ft))),
+              vadd(vmul(sq, (1 - fs) * ft), vmul(tq, fs * ft)));
+}
+void orc_glsl_interpolate_4d(const double* tex, int size_w, int size_z, int size_y, int size_x, const double* idx,
+                             double* out) {
+  store(out, glsl_interpolate_4d(tex, size_w, size_z, size_y, size_x, idx));
+}
+
+/* core/ray-scatter-forward.glsl (normalised, reversed order) */
+static void glsl_ray_scatter_forward(const orc_planet* p, const int* shape, v3 point, v3 direction, v3 l, int above,
+                                     double* idx) {
+  double c[4];
+  scatter_forward(p, shape, point, direction, l, above, c);
+  for (int d = 0; d < 4; d++) idx[3 - d] = c[d] / (shape[d] - 1);
+}
+
+/* atmosphere/transmittance-outer.glsl over the transmittance file's texture (64 x 255 -> 255 wide) */
+void orc_glsl_tiled_transmittance_outer(const orc_planet* p, const int* shape, const double* T, const double* point,
+                                        const double* direction, double* out) {
+  double c[2], idx[2];
+  transmittance_forward(p, shape, arr3(point), arr3(direction), 1, c);
+  idx[0] = c[1] / (shape[1] - 1);
+  idx[1] = c[0] / (shape[0] - 1);
+  store(out, glsl_interpolate_2d(T, shape[0], shape[1], idx));
+}
+
+/* atmosphere/ray-scatter-outer.glsl over the tiled ray-scatter and Mie-strength textures (g: the Mie
+ * phase asymmetry, hard-coded 0.76 in the shader) */
+void orc_glsl_tiled_ray_scatter_outer(const orc_planet* p, const int* shape, const double* S, const double* M, double g,
+                                      const double* light, const double* point, const double* direction, double* out) {
+  double idx[4];
+  v3 l = arr3(light), d = arr3(direction);
+  glsl_ray_scatter_forward(p, shape, arr3(point), d, l, 1, idx);
+  v3 s = glsl_interpolate_4d(S, shape[0], shape[1], shape[2], shape[3], idx);
+  v3 m = glsl_interpolate_4d(M, shape[0], shape[1], shape[2], shape[3], idx);
+  store(out, vadd(s, vmul(m, orc_phase(g, vdot(d, l)))));
+}
+
+/* planet/surface-radiance.glsl over the surface-radiance file's texture */
+void orc_glsl_tiled_surface_radiance(const orc_planet* p, const int* shape, const double* E, const double* point,
+                                     const double* light, double* out) {
+  double c[2], idx[2];
+  surface_radiance_forward(p, shape, arr3(point), arr3(light), c);
+  idx[0] = c[1] / (shape[1] - 1);
+  idx[1] = c[0] / (shape[0] - 1);
+  store(out, glsl_interpolate_2d(E, shape[0], shape[1], idx));
 }
 
 /* The analytic ray-scatter of t_planet.clj:312-315, integrated by surface-radiance over a

@@ -54,7 +54,7 @@ class Params(ctypes.Structure):
     _fields_ = [("planet", Planet), ("mie", Scatter), ("rayleigh", Scatter), ("intensity", ctypes.c_double * 3),
                 ("transmittance_shape", ctypes.c_int * 2), ("surface_radiance_shape", ctypes.c_int * 2),
                 ("scatter_shape", ctypes.c_int * 4), ("ray_steps", ctypes.c_int), ("sphere_steps", ctypes.c_int),
-                ("iterations", ctypes.c_int)]
+                ("iterations", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 # atmosphere_lut.clj:20-40

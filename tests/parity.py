@@ -42,3 +42,31 @@ def oracle_params(oracle, P):
                          P.rayleigh.scatter_g)
     return oracle.params(planet, mie, ray, tuple(P.transmittance_shape), tuple(P.surface_radiance_shape),
                          tuple(P.scatter_shape), P.ray_steps, P.sphere_steps, P.iterations, tuple(P.intensity))
+
+
+def tiled_index(shape, flat):
+    """Position in the 4D->2D tiled .scatter layout (image.clj:299-312: row h*S + s, column e*A + a) of
+    the grid-order flat index of an [h][e][s][a] table entry."""
+    NH, NE, NL, NA = shape
+    h, e, s, a = np.unravel_index(np.asarray(flat), (NH, NE, NL, NA))
+    return (h * NL + s) * (NE * NA) + e * NA + a
+
+
+def fixture_errors(ref, tables, shape):
+    """{table: report} of the fixture's stored entries against `tables` (the four final tables in file
+    layout, keyed like the fixture); 4D tables also per entry category (ray, horizon, clamped, uniform)."""
+    out = {}
+    for name, tab in tables.items():
+        idx = ref[name + "/index"]
+        pos = tiled_index(shape, idx) if name in ("ray-scatter", "mie-strength") else idx
+        got = np.asarray(tab).reshape(-1, 3)[pos]
+        r = report(name, got, ref[name + "/value"], ref[name + "/channel_max"])
+        if name + "/kind" in ref:
+            kind = ref[name + "/kind"]
+            r["by_kind"] = {k: report(name, got[kind & bit != 0], ref[name + "/value"][kind & bit != 0],
+                                      ref[name + "/channel_max"])["max_rel"]
+                            for k, bit in (("ray", 1), ("horizon", 2), ("clamped", 4), ("uniform", 8))
+                            if np.any(kind & bit)}
+        r["entries"] = int(idx.size)
+        out[name] = r
+    return out

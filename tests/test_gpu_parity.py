@@ -18,7 +18,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.parity import TOL_SCATTER, TOL_TRANSMITTANCE, assert_close, oracle_params, rel_errors
+from tests.parity import TOL_SCATTER, TOL_TRANSMITTANCE, assert_close, fixture_errors, oracle_params, rel_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -174,6 +174,44 @@ def test_generate_luts_small(atm, gpu, oracle, iterations, brightness):
 
 
 TABLES = ["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]
+
+
+def test_tiled_tables_read_through_the_renderer_path(atm, gpu, oracle):
+    """The GPU's final tables in file layout, read the way the renderer reads them (the oracle's restatement
+    of transmittance-outer.glsl, ray-scatter-outer.glsl and surface-radiance.glsl over the tiled textures),
+    against the oracle's own tables read the same way, at random points, views and sun directions."""
+    P = small_params(atm, iterations=2)
+    OP = oracle_params(oracle, P)
+    T, E, S, M = oracle.generate_atmosphere_luts(OP)
+    got = gpu.generate_atmosphere_luts(P)
+    shape = tuple(P.scatter_shape)
+    St, Mt = oracle.convert_4d_to_2d(S).reshape(-1, 3), oracle.convert_4d_to_2d(M).reshape(-1, 3)
+    gS, gM = got["ray-scatter"].reshape(-1, 3), got["mie-strength"].reshape(-1, 3)
+    rng = np.random.default_rng(31)
+    R, H = P.planet.radius, P.planet.height
+    res = {"T": [], "S": [], "E": []}
+    for _ in range(300):
+        point = np.array([R + rng.uniform(0, H), 0, 0])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        l = rng.normal(size=3)
+        l /= np.linalg.norm(l)
+        if oracle.is_above_horizon(OP.planet, point, d):
+            res["T"].append((oracle.glsl_tiled_transmittance_outer(OP.planet, P.transmittance_shape,
+                                                                   got["transmittance"].reshape(-1, 3), point, d),
+                             oracle.glsl_tiled_transmittance_outer(OP.planet, P.transmittance_shape, T.reshape(-1, 3),
+                                                                   point, d)))
+            res["S"].append((oracle.glsl_tiled_ray_scatter_outer(OP.planet, shape, gS, gM, 0.76, l, point, d),
+                             oracle.glsl_tiled_ray_scatter_outer(OP.planet, shape, St, Mt, 0.76, l, point, d)))
+        res["E"].append((oracle.glsl_tiled_surface_radiance(OP.planet, P.surface_radiance_shape,
+                                                            got["surface-radiance"].reshape(-1, 3), point, l),
+                         oracle.glsl_tiled_surface_radiance(OP.planet, P.surface_radiance_shape, E.reshape(-1, 3),
+                                                            point, l)))
+    for key, tol in (("T", TOL_TRANSMITTANCE), ("S", TOL_SCATTER), ("E", TOL_SCATTER)):
+        g = np.array([a for a, _ in res[key]])
+        o = np.array([b for _, b in res[key]])
+        assert len(g) > 50
+        assert_close(key, g, o, tol)
 
 
 def test_build_fetch_matches_generate(atm, gpu):
@@ -359,15 +397,56 @@ def test_fetch_before_build_is_an_error(atm):
 
 
 # ------------------------------------------------------------------ full-size end to end
-def test_full_build_against_oracle_fixture(atm, gpu):
-    """Seeded subset of the FP64 oracle's full 5-order build (tests/golden/make_oracle_subsets.py)."""
-    path = os.path.join(HERE, "golden", "oracle_full-5-order.npz")
+TOLS = {"transmittance": TOL_TRANSMITTANCE, "surface-radiance": TOL_SCATTER, "ray-scatter": TOL_SCATTER,
+        "mie-strength": TOL_SCATTER}
+
+
+def _fixture(name):
+    path = os.path.join(HERE, "golden", f"oracle_{name}.npz")
     if not os.path.exists(path):
-        pytest.skip("oracle fixture not generated")
-    ref = np.load(path)
-    luts = gpu.generate_atmosphere_luts(atm.workload("full-5-order"))
-    for name, tol in (("transmittance", TOL_TRANSMITTANCE), ("surface-radiance", TOL_SCATTER),
-                      ("ray-scatter", TOL_SCATTER), ("mie-strength", TOL_SCATTER)):
-        idx = ref[name + "/index"]
-        assert_close(name, luts[name].reshape(-1, 3)[idx], ref[name + "/value"], tol,
-                     channel_max=ref[name + "/channel_max"])
+        pytest.skip(f"oracle fixture {path} not generated")
+    return np.load(path)
+
+
+@pytest.mark.parametrize("workload", ["full-5-order", "production"])
+def test_full_build_against_oracle_fixture(atm, gpu, workload):
+    """The whole 5-order build at full size against the FP64 oracle's build of the same workload
+    (tests/golden/make_oracle_subsets.py): every transmittance and surface-radiance entry, and of the 4D
+    tables every view ray at five (sun elevation, heading) pairs, every horizon-row entry, a sample of the
+    entries whose sun direction the reference clamps, and a uniform sample.  The benchmark's path (the
+    captured graph replayed by sfsim_atmo_build) equals the eager kernel-by-kernel build bit for bit."""
+    ref = _fixture(workload)
+    P = atm.workload(workload)
+    eager = gpu.generate_atmosphere_luts(P)
+    gpu.build(P)
+    gpu.build(P)  # the replay
+    for which, name in enumerate(TABLES):
+        assert np.array_equal(gpu.fetch(P, which), eager[name]), name
+    errs = fixture_errors(ref, eager, tuple(P.scatter_shape))
+    for name, r in errs.items():
+        print(workload, name, r)
+        assert r["max_rel"] <= TOLS[name], (name, r)
+    assert errs["ray-scatter"]["entries"] >= 4 * P.scatter_shape[0] * P.scatter_shape[1]
+
+
+def test_production_scatter_files(atm, gpu, tmp_path):
+    """atmosphere_lut.clj:102-105 at the production shapes: the four files have the sizes the loader's
+    constants imply (atmosphere.clj:572-579; little-endian FP32, no header) and hold, at the oracle
+    fixture's entries, the oracle's values in the 4D->2D tiling the loader uploads (image.clj:299-312,
+    texture.clj:380-398)."""
+    ref = _fixture("production")
+    P = atm.workload("production")
+    luts = gpu.generate_atmosphere_luts(P)
+    atm.write_scatter_files(str(tmp_path), luts)
+    sizes = {"transmittance": 64 * 255, "surface-radiance": 16 * 63, "ray-scatter": 32 * 127 * 32 * 8,
+             "mie-strength": 32 * 127 * 32 * 8}
+    files = {}
+    for name, n in sizes.items():
+        raw = (tmp_path / f"{name}.scatter").read_bytes()
+        assert len(raw) == n * 3 * 4, name
+        files[name] = np.frombuffer(raw, dtype="<f4")
+        assert np.array_equal(files[name], luts[name].reshape(-1)), name
+    # the ray-scatter texture: width heading x elevation = 1016, height sun elevation x height = 1024
+    assert luts["ray-scatter"].shape == (32 * 32, 127 * 8, 3)
+    for name, r in fixture_errors(ref, files, (32, 127, 32, 8)).items():
+        assert r["max_rel"] <= TOLS[name], (name, r)

@@ -3,6 +3,7 @@ and its host-only entry points behave (no GPU compute is called here)."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -77,3 +78,35 @@ def test_stage_names():
 def test_entries_count_of_workloads():
     assert atm.entries(atm.workload("full-5-order")) == 64 * 256 + 16 * 64 + 2 * 32 * 64 * 32 * 8
     assert atm.entries(atm.workload("production")) == 2098096
+
+
+PARAM_FIELDS = [  # (C member designator, bytes) in declaration order
+    ("planet.centre", 24), ("planet.radius", 8), ("planet.height", 8), ("planet.brightness", 24),
+    ("mie.scatter_base", 24), ("mie.scatter_scale", 8), ("mie.scatter_quotient", 8), ("mie.scatter_g", 8),
+    ("rayleigh.scatter_base", 24), ("rayleigh.scatter_scale", 8), ("rayleigh.scatter_quotient", 8),
+    ("rayleigh.scatter_g", 8), ("intensity", 24), ("transmittance_shape", 8), ("surface_radiance_shape", 8),
+    ("scatter_shape", 16), ("ray_steps", 4), ("sphere_steps", 4), ("iterations", 4), ("reserved", 4)]
+
+
+def test_params_struct_has_no_padding(tmp_path):
+    """sfsim_atmo_params as the C compiler lays it out equals the packed field list that coffi's
+    [::mem/struct ...] and ctypes describe: every offset is the sum of the sizes before it and the size is
+    their total (a multiple of 8, so arrays of parameter sets stride alike)."""
+    src = tmp_path / "probe.c"
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{ROOT}/include/sfsim_atmo.h"', "int main(void) {"]
+    for name, _ in PARAM_FIELDS:
+        lines.append(f'  printf("%zu\\n", offsetof(sfsim_atmo_params, {name}));')
+    lines += ['  printf("%zu\\n", sizeof(sfsim_atmo_params));', "  return 0;", "}"]
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    packed = np.cumsum([0] + [n for _, n in PARAM_FIELDS]).tolist()
+    assert got == packed
+    assert got[-1] == 232 and got[-1] % 8 == 0
+    assert ctypes.sizeof(atm.Params) == 232
+    for (name, _), off in zip(PARAM_FIELDS, packed):
+        outer, _, inner = name.partition(".")
+        base = getattr(atm.Params, outer).offset
+        sub = getattr(type(getattr(atm.Params(), outer)), inner).offset if inner else 0
+        assert base + sub == off, name

@@ -165,8 +165,23 @@ def test_reference_golden(oracle, case):
         table = np.array(case["table"], float)
         shape = list(table.shape)
         idx = [c * (n - 1) for c, n in zip(reversed(case["glsl_idx"]), shape)]
-        got = oracle.interpolate(np.repeat(table[..., None], 3, axis=-1).reshape(-1, 3), shape, idx)
+        rgb = np.repeat(table[..., None], 3, axis=-1)
+        got = oracle.interpolate(rgb.reshape(-1, 3), shape, idx)
         assert abs(got[0] - case["expected"]) <= tol
+        # the same value through the shader's texture path over the file layout (4D tables tiled)
+        if len(shape) == 4:
+            tex = oracle.convert_4d_to_2d(rgb)
+            got = oracle.glsl_interpolate_4d(tex.reshape(-1, 3), *shape, case["glsl_idx"])
+        else:
+            got = oracle.glsl_interpolate_2d(rgb.reshape(-1, 3), shape[0], shape[1], case["glsl_idx"])
+        assert abs(got[0] - case["expected"]) <= tol
+    elif fn == "glsl_convert_2d_index":
+        got = oracle.glsl_convert_2d_index(case["idx"], case["size_y"], case["size_x"])
+        assert np.max(np.abs(got - np.array(case["expected"]))) <= tol
+    elif fn == "glsl_make_2d_index_from_4d":
+        got = oracle.glsl_make_2d_index_from_4d(case["idx"], *case["sizes_wzyx"])
+        sel = got[:2] if case["selector"] == "st" else got[2:]
+        assert np.max(np.abs(sel - np.array(case["expected"]))) <= tol
     elif fn.startswith("table_"):
         t = _size12_tables(oracle)
         earth = t["earth"]
@@ -174,11 +189,16 @@ def test_reference_golden(oracle, case):
             got = oracle.glsl_transmittance_track(earth, (12, 12), t["T"], case["p"], case["q"])
         elif fn == "table_transmittance_outer":
             got = oracle.glsl_transmittance_outer(earth, (12, 12), t["T"], case["p"], case["d"])
+            # and through the texture path over the file layout
+            tiled = oracle.glsl_tiled_transmittance_outer(earth, (12, 12), t["T"], case["p"], case["d"])
+            assert abs(tiled[case["channel"]] - case["expected"]) <= tol
         elif fn == "table_ray_scatter_track":
             got = oracle.glsl_ray_scatter_track(earth, (12, 12, 12, 12), t["S"], t["S"], (12, 12), t["T"], 0.76,
                                                 case["light"], case["p"], case["q"])
         else:
             got = oracle.glsl_surface_radiance(earth, (12, 12), t["E"], case["p"], case["light"])
+            tiled = oracle.glsl_tiled_surface_radiance(earth, (12, 12), t["E"], case["p"], case["light"])
+            assert abs(tiled[case["channel"]] - case["expected"]) <= tol
         assert abs(got[case["channel"]] - case["expected"]) <= tol
     else:
         pytest.fail(f"unknown golden function {fn}")
