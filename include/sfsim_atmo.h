@@ -200,6 +200,16 @@ int sfsim_atmo_read_scatter(const char* path, float* data, size_t count);
  * given steps (sphere.clj:70-105). */
 int sfsim_atmo_sphere_directions(int half, int steps);
 
+/* The standard atmosphere that sfsim's aerodynamics reads (atmosphere.clj:425-506, after Hull,
+ * "Fundamentals of Airplane Flight Mechanics"): temperature (K), pressure (Pa) and density (kg/m^3) at a
+ * height above sea level (m), from the troposphere formulas up to 36089 ft, the lower-stratosphere ones up to
+ * 65617 ft and the upper-stratosphere ones above (temperature-at-height, pressure-at-height,
+ * density-at-height).  Any output may be NULL; a non-finite height is SFSIM_ATMO_ERR_NONFINITE.  Host code:
+ * the reference's callers evaluate it one flight state at a time (aerodynamics.clj:424). */
+int sfsim_atmo_standard_atmosphere(double height, double* temperature, double* pressure, double* density);
+/* speed-of-sound (atmosphere.clj:508-511): 331.3 sqrt(T / 273.15) m/s for a temperature T in K. */
+double sfsim_atmo_speed_of_sound(double temperature);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,7 @@ EXPORTS = (
     "sfsim_atmo_surrogate_weights", "sfsim_atmo_surrogate_set_weights", "sfsim_atmo_surrogate_eval",
     "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_build_from_sums", "sfsim_atmo_order_sums",
     "sfsim_atmo_surrogate_predict", "sfsim_atmo_set_sun_table", "sfsim_atmo_get_sun_table",
+    "sfsim_atmo_standard_atmosphere", "sfsim_atmo_speed_of_sound",
 )
 
 
@@ -150,6 +151,10 @@ def lib():
         L.sfsim_atmo_write_scatter.argtypes = [ctypes.c_char_p, fp, ctypes.c_size_t]
         L.sfsim_atmo_read_scatter.argtypes = [ctypes.c_char_p, fp, ctypes.c_size_t]
         L.sfsim_atmo_sphere_directions.argtypes = [ctypes.c_int, ctypes.c_int]
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.sfsim_atmo_standard_atmosphere.argtypes = [ctypes.c_double, dp, dp, dp]
+        L.sfsim_atmo_speed_of_sound.argtypes = [ctypes.c_double]
+        L.sfsim_atmo_speed_of_sound.restype = ctypes.c_double
         L.sfsim_atmo_surrogate_size.argtypes = []
         L.sfsim_atmo_surrogate_size.restype = ctypes.c_size_t
         L.sfsim_atmo_surrogate_reset.argtypes = [vp, ctypes.c_ulonglong, ctypes.c_int]
@@ -187,6 +192,21 @@ def _f32(a, entries_):
 
 def sphere_directions(half, steps):
     return lib().sfsim_atmo_sphere_directions(int(half), steps)
+
+
+def standard_atmosphere(height):
+    """(temperature K, pressure Pa, density kg/m^3) at a height in m: temperature-at-height,
+    pressure-at-height and density-at-height of atmosphere.clj:425-506."""
+    t, p, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = lib().sfsim_atmo_standard_atmosphere(float(height), ctypes.byref(t), ctypes.byref(p), ctypes.byref(d))
+    if rc:
+        raise AtmoError(rc, f"height {height} is not finite")
+    return t.value, p.value, d.value
+
+
+def speed_of_sound(temperature):
+    """speed-of-sound of atmosphere.clj:508-511 (m/s at a temperature in K)."""
+    return lib().sfsim_atmo_speed_of_sound(float(temperature))
 
 
 def write_scatter(path, data):

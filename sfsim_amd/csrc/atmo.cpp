@@ -1413,6 +1413,41 @@ int sfsim_atmo_read_scatter(const char* path, float* data, size_t count) {
   return (r == count && extra == EOF) ? 0 : SFSIM_ATMO_ERR_IO;
 }
 
+// Units of the reference (units.clj:14-19): degrees Rankine, feet, pounds-force, slugs.
+namespace {
+constexpr double kRankine = 5.0 / 9.0, kFoot = 0.3048, kPoundForce = 0.45359237 * 9.8066500286389,
+                 kSlug = 14.5939029372;
+double isa_temperature(double feet) {  // atmosphere.clj:425-450
+  if (feet <= 36089.0) return (518.69 - 3.5662e-3 * feet) * kRankine;
+  if (feet <= 65617.0) return 389.99 * kRankine;
+  return (389.99 + 5.4864e-4 * (feet - 65617.0)) * kRankine;
+}
+}  // namespace
+
+int sfsim_atmo_standard_atmosphere(double height, double* temperature, double* pressure, double* density) {
+  if (!std::isfinite(height)) return SFSIM_ATMO_ERR_NONFINITE;
+  const double ft = height / kFoot, psf = kPoundForce / (kFoot * kFoot), spcf = kSlug / (kFoot * kFoot * kFoot);
+  const double t = isa_temperature(ft), tr = t / kRankine;
+  double pr, rho;
+  if (ft <= 36089.0) {  // atmosphere.clj:453-456, 480-483
+    pr = 1.1376e-11 * std::pow(tr, 5.2560) * psf;
+    rho = 6.6277e-15 * std::pow(tr, 4.2560) * spcf;
+  } else if (ft <= 65617.0) {  // atmosphere.clj:459-462, 486-489: density from the pressure in lbf / ft^2
+    const double p_psf = 2678.4 * std::exp(-4.8063e-5 * ft);
+    pr = p_psf * psf;
+    rho = 1.4939e-6 * p_psf * spcf;
+  } else {  // atmosphere.clj:465-468, 492-494
+    pr = 3.7930e+90 * std::pow(tr, -34.164) * psf;
+    rho = 2.2099e+87 * std::pow(tr, -35.164) * spcf;
+  }
+  if (temperature) *temperature = t;
+  if (pressure) *pressure = pr;
+  if (density) *density = rho;
+  return 0;
+}
+
+double sfsim_atmo_speed_of_sound(double temperature) { return 331.3 * std::sqrt(temperature / 273.15); }
+
 int sfsim_atmo_sphere_directions(int half, int steps) {
   int theta_steps = half ? (steps >> 2) : (steps >> 1);
   if (theta_steps < 1) return 0;
