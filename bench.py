@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--workload", default=WORKLOAD)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-seeded", action="store_true", help="skip the surrogate-seeded leg")
+    ap.add_argument("--batch", type=int, default=16,
+                    help="atmospheres of the batched-build leg (sfsim_atmo_build_batch); 0 skips it")
     ap.add_argument("--train-steps", type=int, default=24000)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="oracle threads (default: the CPUs this process may use, capped by OMP_NUM_THREADS)")
@@ -162,6 +164,24 @@ def seeded_leg(atm, ctx, P, workload, steps, train_steps):
     return out
 
 
+def batch_leg(atm, ctx, P, count, steps):
+    """The batched build (sfsim_atmo_build_batch): `count` atmospheres drawn from the training ranges on the
+    workload's grid in one launch sequence per build, after one untimed warm-up build; throughput per
+    atmosphere and for the whole batch."""
+    rng = np.random.default_rng(20261203)
+    family = [P] + [atm.random_atmosphere(rng, P) for _ in range(count - 1)]
+    ctx.build_batch(family)
+    reps = max(1, min(steps, 10))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.build_batch(family)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    n_entries = atm.entries(P) * count
+    return {"atmospheres": count, "builds": reps, "ms_per_batch": round(ms, 3),
+            "ms_per_atmosphere": round(ms / count, 4), "value": round(n_entries / (ms / 1e3), 1),
+            "unit": "entries/s"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -222,6 +242,8 @@ def main():
         for m in ("m1", "m2"):
             seeded[m]["speedup_vs_plain"] = round(ms_per_step / seeded[m]["ms_per_step"], 3)
 
+    batch = batch_leg(atm, ctx, P, args.batch, args.steps) if args.batch > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         host = host_info()
@@ -253,6 +275,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "surrogate_seeded": seeded,
+        "batched": batch,
     }
     ctx.close()
     print(json.dumps(line), flush=True)

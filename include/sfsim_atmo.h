@@ -144,6 +144,18 @@ int sfsim_atmo_build(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p);
 int sfsim_atmo_build_timed(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p);
 int sfsim_atmo_fetch(sfsim_atmo_ctx* ctx, int which, float* out, size_t count);
 
+/* generate-atmosphere-luts for `count` atmospheres at once: the reference builds one atmosphere at a time,
+ * parallel only over the first table dimension (pmap, interpolate.clj:72); here every launch of the
+ * pipeline covers the entries of all the atmospheres (grid = entries x atmospheres).  The atmospheres must
+ * share the planet's radius and height, the table shapes, the step counts and the iterations; their
+ * scattering components, brightness and intensity may differ.  The geometry that depends only on the planet
+ * and the grid is computed once; every other table exists once per atmosphere in context-owned device
+ * memory.  Needs the sun optical-depth table (sfsim_atmo_set_sun_table with rows > 0, the default).
+ * sfsim_atmo_fetch_batch copies one final table of one atmosphere out (which and layouts as
+ * sfsim_atmo_fetch).  Results equal those of sfsim_atmo_generate_luts for each atmosphere alone. */
+int sfsim_atmo_build_batch(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atmospheres, int count);
+int sfsim_atmo_fetch_batch(sfsim_atmo_ctx* ctx, int atmosphere, int which, float* out, size_t count);
+
 /* Kernel durations of the last sfsim_atmo_build_timed (or seeded build), measured with HIP events
  * on the context stream and summed per kernel kind: ms[k] milliseconds and launches[k] launches of kind k, for
  * k < min(n, number of kinds); returns the number of kinds.  Kind names: sfsim_atmo_stage_name(k). */

@@ -31,6 +31,7 @@ EXPORTS = (
     "sfsim_atmo_surrogate_weights", "sfsim_atmo_surrogate_set_weights", "sfsim_atmo_surrogate_eval",
     "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_build_from_sums", "sfsim_atmo_order_sums",
     "sfsim_atmo_surrogate_predict", "sfsim_atmo_set_sun_table", "sfsim_atmo_get_sun_table",
+    "sfsim_atmo_build_batch", "sfsim_atmo_fetch_batch",
     "sfsim_atmo_standard_atmosphere", "sfsim_atmo_speed_of_sound",
 )
 
@@ -143,6 +144,8 @@ def lib():
         L.sfsim_atmo_build.argtypes = [vp, pp]
         L.sfsim_atmo_build_timed.argtypes = [vp, pp]
         L.sfsim_atmo_fetch.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
+        L.sfsim_atmo_build_batch.argtypes = [vp, pp, ctypes.c_int]
+        L.sfsim_atmo_fetch_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_size_t]
         L.sfsim_atmo_stage_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                                                ctypes.c_int]
         L.sfsim_atmo_stage_name.argtypes = [ctypes.c_int]
@@ -344,6 +347,23 @@ class Atmosphere:
                   (s[0] * s[2], s[1] * s[3], 3), (s[0] * s[2], s[1] * s[3], 3)]
         out = np.zeros(shapes[which], np.float32)
         self._check(lib().sfsim_atmo_fetch(self._ctx, which, _fp(out), out.size))
+        return out
+
+    def build_batch(self, atmospheres):
+        """sfsim_atmo_build_batch: the LUT sets of several atmospheres (same planet radius and height, shapes,
+        steps and iterations) in one launch sequence, kept on the device; read with fetch_batch."""
+        arr = (Params * len(atmospheres))(*atmospheres)
+        self._check(lib().sfsim_atmo_build_batch(self._ctx, arr, len(atmospheres)))
+
+    def fetch_batch(self, P, atmosphere, which):
+        """Final table `which` (0 transmittance, 1 surface radiance, 2 ray scatter, 3 Mie strength) of
+        atmosphere `atmosphere` of the last build_batch, shaped as fetch's."""
+        s = P.scatter_shape
+        shapes = [(P.transmittance_shape[0], P.transmittance_shape[1], 3),
+                  (P.surface_radiance_shape[0], P.surface_radiance_shape[1], 3),
+                  (s[0] * s[2], s[1] * s[3], 3), (s[0] * s[2], s[1] * s[3], 3)]
+        out = np.zeros(shapes[which], np.float32)
+        self._check(lib().sfsim_atmo_fetch_batch(self._ctx, atmosphere, which, _fp(out), out.size))
         return out
 
     def stage_timings(self):

@@ -57,6 +57,9 @@ struct sfsim_atmo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string error;
+  // set on the stack context of a batched build: every buffer is pre-assigned in the batch's memory, and a
+  // request that does not fit is an error instead of a hipMalloc
+  bool no_alloc = false;
   // tables (float4 per entry)
   DeviceBuffer T, dE[2], Eacc[2], R1, M1, dJ, dS[2], Sacc[2];
   // packed outputs (RGB floats): 0 transmittance, 1 surface radiance, 2 ray scatter (tiled), 3 mie (tiled)
@@ -96,6 +99,14 @@ struct sfsim_atmo_ctx {
   unsigned long long seeded_gen = 0;
   int seeded_m = 0;
   const void* seeded_bufs[3] = {nullptr, nullptr, nullptr};
+  // batched builds: one block of buffers per atmosphere (batch_stride bytes apart) after the shared
+  // geometry, the per-atmosphere parameters on the device, and the final tables of atmosphere 0
+  DeviceBuffer batch_mem, batch_params;
+  int batch_count = 0;
+  size_t batch_stride = 0;
+  void* batch_out[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t batch_out_count[4] = {0, 0, 0, 0};
+  bool batch_built = false;
   // timing
   std::vector<hipEvent_t> events;
   std::vector<int> event_kind;
@@ -129,6 +140,8 @@ int hip_fail(sfsim_atmo_ctx* ctx, const char* what, hipError_t e) {
 
 int ensure(sfsim_atmo_ctx* ctx, DeviceBuffer& b, size_t bytes) {
   if (b.bytes >= bytes && b.ptr) return 0;
+  if (ctx->no_alloc)
+    return fail(ctx, SFSIM_ATMO_ERR_ALLOC, "a build requested a buffer outside its preassigned layout");
   ctx->alloc_gen++;  // captured graphs hold buffer addresses
   if (b.ptr) {
     (void)hipFree(b.ptr);
@@ -639,7 +652,10 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
     jobs[nj++] = ResampleJob{4, S_in, nullptr, nullptr, (float*)ctx->out[2].ptr, 1};
     LAUNCH(timer, kFinal, launch_resamples(P, jobs, nj, ctx->stream));
   }
-  if (!E_cur) LAUNCH(timer, kFinal, launch_fill_zero((float*)ctx->out[1].ptr, (long long)ctx->out_count[1], ctx->stream));
+  if (!E_cur)  // every atmosphere's surface-radiance table (blocks atm_stride apart in a batch)
+    for (int a = 0; a < std::max(P.batch, 1); ++a)
+      LAUNCH(timer, kFinal, launch_fill_zero((float*)((char*)ctx->out[1].ptr + (size_t)a * P.atm_stride),
+                                             (long long)ctx->out_count[1], ctx->stream));
   return 0;
 }
 
@@ -949,6 +965,73 @@ float4* stage_out(sfsim_atmo_ctx* ctx, size_t entries, int* rc) {
   return (float4*)ctx->stage_out4.ptr;
 }
 
+// Carves buffers out of device memory at `base` (256-byte aligned offsets); with base == nullptr only the
+// size is counted.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  void take(DeviceBuffer& b, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    if (base) {
+      b.ptr = base + off;
+      b.bytes = bytes;
+    }
+    off += (bytes + 255) & ~(size_t)255;
+  }
+};
+
+// The buffers of a build that depend only on the planet and the grid (one set for all the atmospheres
+// of a batch): ray order, re-sampling and sun maps, quadrature directions, half-sphere geometry.
+size_t layout_shared(sfsim_atmo_ctx* c, const sfsim_atmo_params* p, char* base) {
+  Carver k{base};
+  const DevParams P = make_dev_params(p);
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  k.take(c->ray_order, (size_t)p->scatter_shape[0] * p->scatter_shape[1] * sizeof(int));
+  k.take(c->rmap, resample_map_bytes(P));
+  k.take(c->sunmap, (2 * sizeof(float4) + sizeof(double2)) * nS);
+  if (p->iterations > 0) {
+    const int sn = sfsim_atmo_sphere_directions(0, p->sphere_steps), hn = sfsim_atmo_sphere_directions(1, p->ray_steps);
+    k.take(c->sphere_dirs, (size_t)sn * sizeof(Dir));
+    k.take(c->half_dirs, (size_t)hn * sizeof(Dir));
+    k.take(c->sgeo, (size_t)p->surface_radiance_shape[0] * hn * sizeof(float4));
+  }
+  return k.off;
+}
+
+// The buffers of one atmosphere's build that prepare_pipeline asks for (with_outputs: also the four packed
+// final tables); the same layout for every atmosphere of a batch, one block each.
+size_t layout_atmosphere(sfsim_atmo_ctx* c, const sfsim_atmo_params* p, char* base, bool with_outputs) {
+  Carver k{base};
+  const DevParams P = make_dev_params(p);
+  const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  if (c->sun_rows > 0) k.take(c->sun_depth, (size_t)c->sun_rows * c->sun_cols * sizeof(float2));
+  k.take(c->ray_geo, ray_geometry_bytes(P));
+  if (p->iterations > 0) {
+    const int sn = sfsim_atmo_sphere_directions(0, p->sphere_steps);
+    k.take(c->geom, direction_geometry_bytes(P, sn));
+    const size_t slab = nS / (size_t)p->scatter_shape[1] * sn * sizeof(float4);
+    k.take(c->slab, slab);
+    k.take(c->slab_mie, slab);
+    for (int i = 0; i < 2; ++i) k.take(c->Ssum[i], nS * sizeof(float4));
+  }
+  k.take(c->T, nT * sizeof(float4));
+  for (int i = 0; i < 2; ++i) {
+    k.take(c->dE[i], nE * sizeof(float4));
+    k.take(c->Eacc[i], nE * sizeof(float4));
+    k.take(c->dS[i], nS * sizeof(float4));
+    k.take(c->Sacc[i], nS * sizeof(float4));
+  }
+  k.take(c->R1, nS * sizeof(float4));
+  k.take(c->M1, nS * sizeof(float4));
+  k.take(c->dJ, nS * sizeof(float4));
+  if (with_outputs) {
+    const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
+    for (int i = 0; i < 4; ++i) k.take(c->out[i], outc[i] * sizeof(float));
+  }
+  return k.off;
+}
+
 }  // namespace
 
 extern "C" {
@@ -987,7 +1070,8 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
                           &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sunmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum[0], &ctx->Ssum[1]};
+                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sunmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum[0], &ctx->Ssum[1],
+                          &ctx->batch_mem, &ctx->batch_params};
   for (DeviceBuffer* b : more) release(*b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
@@ -1174,6 +1258,91 @@ int sfsim_atmo_generate_luts(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, fl
   float* outs[4] = {transmittance, surface_radiance, ray_scatter, mie_strength};
   for (int i = 0; i < 4; ++i)
     if (outs[i]) TRY(sfsim_atmo_fetch(ctx, i, outs[i], ctx->out_count[i]));
+  return 0;
+}
+
+int sfsim_atmo_build_batch(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* ps, int count) {
+  TRY(activate(ctx));
+  if (!ps || count < 1 || count > 65535) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "count must be in [1, 65535]");
+  const sfsim_atmo_params* p0 = &ps[0];
+  for (int a = 0; a < count; ++a) {
+    const sfsim_atmo_params* p = &ps[a];
+    TRY(validate(ctx, p));
+    const bool same =
+        p->planet.radius == p0->planet.radius && p->planet.height == p0->planet.height &&
+        std::memcmp(p->transmittance_shape, p0->transmittance_shape, sizeof(p->transmittance_shape)) == 0 &&
+        std::memcmp(p->surface_radiance_shape, p0->surface_radiance_shape, sizeof(p->surface_radiance_shape)) == 0 &&
+        std::memcmp(p->scatter_shape, p0->scatter_shape, sizeof(p->scatter_shape)) == 0 &&
+        p->ray_steps == p0->ray_steps && p->sphere_steps == p0->sphere_steps && p->iterations == p0->iterations;
+    if (!same)
+      return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT,
+                  "the atmospheres of a batch must share the planet's radius and height, the table shapes, the step "
+                  "counts and the iterations");
+  }
+  if (ctx->sun_rows <= 0)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "batched builds read the sun optical depth from its table (rows > 0)");
+  if (p0->iterations > 0) {
+    TRY(validate_sphere(ctx, p0->sphere_steps));
+    TRY(validate_half_sphere(ctx, p0->ray_steps));
+  }
+  ctx->batch_built = false;
+  // the shared geometry, then one block of buffers per atmosphere
+  sfsim_atmo_ctx sh;
+  sh.device = ctx->device;
+  sh.stream = ctx->stream;
+  sh.no_alloc = true;
+  sh.sun_rows = ctx->sun_rows;
+  sh.sun_cols = ctx->sun_cols;
+  const size_t shared = layout_shared(&sh, p0, nullptr), block = layout_atmosphere(&sh, p0, nullptr, true);
+  TRY(ensure(ctx, ctx->batch_mem, shared + (size_t)count * block));
+  TRY(ensure(ctx, ctx->batch_params, (size_t)count * sizeof(DevParams)));
+  char* base = (char*)ctx->batch_mem.ptr;
+  layout_shared(&sh, p0, base);
+  layout_atmosphere(&sh, p0, base + shared, true);
+  auto fail_sh = [&](int rc) {
+    ctx->error = sh.error;
+    return rc;
+  };
+  int rc = prepare_pipeline(&sh, p0);
+  if (rc) return fail_sh(rc);
+  // per-atmosphere parameters: each atmosphere's own ray geometry and sun table, the shared maps
+  const DevParams* dparams = (const DevParams*)ctx->batch_params.ptr;
+  std::vector<DevParams> host(count);
+  for (int a = 0; a < count; ++a) {
+    DevParams Pa = dev_params(&sh, &ps[a]);
+    Pa.ray_geo = (char*)sh.ray_geo.ptr + (size_t)a * block;
+    Pa.sun_depth = (float2*)((char*)sh.sun_depth.ptr + (size_t)a * block);
+    Pa.batch = count;
+    Pa.atm_stride = block;
+    Pa.batch_params = dparams;
+    host[a] = Pa;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->batch_params.ptr, host.data(), (size_t)count * sizeof(DevParams),
+                              hipMemcpyHostToDevice, ctx->stream));
+  Timer timer{&sh, false};
+  rc = enqueue_pipeline(&sh, p0, host[0], nullptr, timer, nullptr, true);
+  if (rc) return fail_sh(rc);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->batch_count = count;
+  ctx->batch_stride = block;
+  for (int i = 0; i < 4; ++i) {
+    ctx->batch_out[i] = sh.out[i].ptr;
+    ctx->batch_out_count[i] = sh.out_count[i];
+  }
+  ctx->batch_built = true;
+  return 0;
+}
+
+int sfsim_atmo_fetch_batch(sfsim_atmo_ctx* ctx, int atmosphere, int which, float* out, size_t count) {
+  TRY(activate(ctx));
+  if (!ctx->batch_built) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "no completed batched build on this context");
+  if (atmosphere < 0 || atmosphere >= ctx->batch_count || which < 0 || which > 3 || !out)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "atmosphere must be in the batch, which 0..3 and out non-NULL");
+  if (count != ctx->batch_out_count[which])
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "count does not match the table size");
+  const char* src = (const char*)ctx->batch_out[which] + (size_t)atmosphere * ctx->batch_stride;
+  HIP_TRY(ctx, hipMemcpyAsync(out, src, count * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return 0;
 }
 

@@ -1,6 +1,8 @@
 // Parameters of one table build as the kernels see them (host fills, passed by value).
 #pragma once
 
+#include <cstdint>
+
 #include "geometry.hpp"
 
 namespace sfsim {
@@ -62,7 +64,32 @@ struct DevParams {
   // order in which the ray-scatter kernel visits the view rays (NH x NE ray indices, ascending impact
   // parameter |x| sqrt(1 - mu^2)); null: grid order
   const int* ray_order;
+  // batched builds (launches over several atmospheres sharing the planet and the grid): batch > 0
+  // atmospheres, atmosphere z = blockIdx.z with parameters batch_params[z] (a device array), and every
+  // per-atmosphere table (and ray_geo, sun_depth) of atmosphere z lies z atm_stride bytes after
+  // atmosphere 0's; the geometry that depends only on the planet and the grid (rmap, sun maps, sgeo,
+  // ray_order, quadrature directions) is shared.  batch == 0: one atmosphere, parameters by value.
+  int batch;
+  size_t atm_stride;
+  const DevParams* batch_params;
 };
+
+// The parameters of the launch's atmosphere and its own copy of a per-atmosphere table.
+template <bool kB>
+__device__ __forceinline__ const DevParams& params_of(const DevParams& P0, const DevParams* __restrict__ PB) {
+  if constexpr (kB)
+    return PB[blockIdx.z];
+  else
+    return P0;
+}
+
+template <bool kB, class T>
+__device__ __forceinline__ T* atm_ptr(T* p, const DevParams& P) {
+  if constexpr (kB)
+    return p ? (T*)((uintptr_t)p + (uintptr_t)blockIdx.z * P.atm_stride) : p;
+  else
+    return p;
+}
 
 __host__ __device__ inline size_t resample_map_bytes(const DevParams& P) {
   return (size_t)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3] * (sizeof(float4) + sizeof(ushort4));

@@ -625,10 +625,21 @@ __global__ void __launch_bounds__(kBlock) ray_geometry_kernel(const DevParams P,
 // The ray's samples (t, h, T(x, p_i), sun-table terms) come from the per-build geometry through LDS.
 constexpr int kSSTBlock = 256;
 
+template <bool kB>
 __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
-    const DevParams P, const double2* __restrict__ geo_th, const float4* __restrict__ geo_T,
-    const float4* __restrict__ geo_ss, const float2* __restrict__ geo_row, float4* __restrict__ R1,
-    float4* __restrict__ M1, WorkCounters* wc) {
+    const DevParams P0, const DevParams* __restrict__ PB, const double2* __restrict__ geo_th,
+    const float4* __restrict__ geo_T, const float4* __restrict__ geo_ss, const float2* __restrict__ geo_row,
+    float4* __restrict__ R1, float4* __restrict__ M1, WorkCounters* wc) {
+  const DevParams& P = params_of<kB>(P0, PB);
+  if constexpr (kB) {  // this atmosphere's tables and ray geometry
+    const RayGeoView V = ray_geo_view(P);
+    geo_th = V.th;
+    geo_T = V.T;
+    geo_ss = V.ss;
+    geo_row = V.row;
+    R1 = atm_ptr<kB>(R1, P);
+    M1 = atm_ptr<kB>(M1, P);
+  }
   const int NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps, cols = P.sun_cols;
@@ -832,12 +843,25 @@ constexpr int kRSHalf = kRSBlock / 2;
 
 // Eight waves per SIMD (four workgroups per CU, as the LDS allows): at most 64 VGPRs, the re-sampling
 // of the accumulations spilling a few registers outside the loops.
-template <int EPT>
-__global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) ray_scatter_kernel(const DevParams P, const float4* __restrict__ dJ,
-                                                               const float4* __restrict__ geo_T,
-                                                               const float4* __restrict__ geo_rs,
-                                                               float4* __restrict__ out, const RsAccum accum, int Q,
-                                                               WorkCounters* wc) {
+template <int EPT, bool kB>
+__global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) ray_scatter_kernel(
+    const DevParams P0, const DevParams* __restrict__ PB, const float4* __restrict__ dJ,
+    const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs, float4* __restrict__ out,
+    const RsAccum accum_arg, int Q, WorkCounters* wc) {
+  const DevParams& P = params_of<kB>(P0, PB);
+  RsAccum accum = accum_arg;
+  if constexpr (kB) {  // this atmosphere's tables and ray geometry
+    const RayGeoView V = ray_geo_view(P);
+    geo_T = V.T;
+    geo_rs = V.rs;
+    dJ = atm_ptr<kB>(dJ, P);
+    out = atm_ptr<kB>(out, P);
+    accum.S_in = atm_ptr<kB>(accum.S_in, P);
+    accum.S_out = atm_ptr<kB>(accum.S_out, P);
+    accum.dE = atm_ptr<kB>(accum.dE, P);
+    accum.E_in = atm_ptr<kB>(accum.E_in, P);
+    accum.E_out = atm_ptr<kB>(accum.E_out, P);
+  }
   // dynamic LDS: 2 x Q x (NL x NA) slab entries (chunk c in half c % 2), then per sample the slab
   // offsets (in units of NL x NA), their bilinear weights and the fold slots whose node differs from the
   // previous sample's
@@ -1026,11 +1050,19 @@ __global__ void __launch_bounds__(kBlock) direction_geometry_kernel(const DevPar
 
 // slab[h][j][s][a] = dS at (height of x_h, elevation coordinate of omega_j) for every (s, a) grid
 // point; with mie != nullptr the Mie strength slab is folded the same way into slab_mie.
-__global__ void __launch_bounds__(kBlock) point_scatter_combine_kernel(const DevParams P, const DirGeom* __restrict__ geom,
-                                                                       int ndirs, const float4* __restrict__ dS,
+template <bool kB>
+__global__ void __launch_bounds__(kBlock) point_scatter_combine_kernel(const DevParams P0, const DevParams* __restrict__ PB,
+                                                                       const DirGeom* __restrict__ geom, int ndirs,
+                                                                       const float4* __restrict__ dS,
                                                                        const float4* __restrict__ mie,
                                                                        float4* __restrict__ slab,
                                                                        float4* __restrict__ slab_mie) {
+  const DevParams& P = params_of<kB>(P0, PB);
+  geom = atm_ptr<kB>(geom, P);
+  dS = atm_ptr<kB>(dS, P);
+  mie = atm_ptr<kB>(mie, P);
+  slab = atm_ptr<kB>(slab, P);
+  slab_mie = atm_ptr<kB>(slab_mie, P);
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int b = xcd_ray(blockIdx.x, NH * ndirs);  // the XCD that reads this height's slabs (same ranges)
@@ -1176,8 +1208,9 @@ __device__ __forceinline__ void surface_radiance_entry(const DevParams& P, int n
 
 // The iteration's surface radiance (it reads the same dS and Mie strength) rides in the launch: the
 // sr.count workgroups after the rays' compute its entries with the LDS-folded kernel body.
-template <bool kMie, bool kELds>
-__global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P, const Dir* __restrict__ dirs,
+template <bool kMie, bool kELds, bool kB>
+__global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P0, const DevParams* __restrict__ PB,
+                                                               const Dir* __restrict__ dirs,
                                                                const DirGeom* __restrict__ geom, int ndirs,
                                                                const float4* __restrict__ slab,
                                                                const float4* __restrict__ slab_mie,
@@ -1185,14 +1218,20 @@ __global__ void __launch_bounds__(kBlock) point_scatter_kernel(const DevParams P
                                                                const SrJob sr, WorkCounters* wc) {
   extern __shared__ DirLds s_dir[];
   float4* s_dE = (float4*)(s_dir + ndirs);
+  const DevParams& P = params_of<kB>(P0, PB);
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   if ((int)blockIdx.x >= NH * NE) {
     __shared__ float3 s_red[kBlock / 64];
-    surface_radiance_entry<true>(P, sr.ndirs, sr.dS, sr.mie, sr.out, wc, (int)blockIdx.x - NH * NE, (float4*)s_dir,
-                                 s_red);
+    surface_radiance_entry<true>(P, sr.ndirs, atm_ptr<kB>(sr.dS, P), atm_ptr<kB>(sr.mie, P), atm_ptr<kB>(sr.out, P),
+                                 wc, (int)blockIdx.x - NH * NE, (float4*)s_dir, s_red);
     return;
   }
+  geom = atm_ptr<kB>(geom, P);
+  slab = atm_ptr<kB>(slab, P);
+  slab_mie = atm_ptr<kB>(slab_mie, P);
+  dE = atm_ptr<kB>(dE, P);
+  out = atm_ptr<kB>(out, P);
   const int rayid = xcd_ray(blockIdx.x, NH * NE);
   if (kELds)
     for (int i = threadIdx.x; i < P.nE[1]; i += blockDim.x) s_dE[i] = dE[i];
@@ -1353,14 +1392,16 @@ __device__ __forceinline__ void surface_radiance_entry(const DevParams& P, int n
   if (threadIdx.x == 0) count(wc, 0, 0, (unsigned long long)ndirs * (mie ? 2 : 1), 0);
 }
 
-template <bool kFold>
-__global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P, int ndirs,
-                                                                  const float4* __restrict__ dS,
+template <bool kFold, bool kB>
+__global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParams P0, const DevParams* __restrict__ PB,
+                                                                  int ndirs, const float4* __restrict__ dS,
                                                                   const float4* __restrict__ mie,
                                                                   float4* __restrict__ out, WorkCounters* wc) {
   __shared__ float3 s_red[kBlock / 64];
   extern __shared__ float4 s_fold[];  // kFold: [NE x NA] dS, then [NE x NA] Mie strength
-  surface_radiance_entry<kFold>(P, ndirs, dS, mie, out, wc, blockIdx.x, s_fold, s_red);
+  const DevParams& P = params_of<kB>(P0, PB);
+  surface_radiance_entry<kFold>(P, ndirs, atm_ptr<kB>(dS, P), atm_ptr<kB>(mie, P), atm_ptr<kB>(out, P), wc,
+                                blockIdx.x, s_fold, s_red);
 }
 
 // f(backward(i)) re-sampling of interpolated tables (interpolate.clj:54-72 with interpolation-table).
@@ -1434,21 +1475,27 @@ struct PrologueArgs {
   unsigned first[8];
 };
 
-__global__ void __launch_bounds__(kBlock) prologue_kernel(const DevParams P, const PrologueArgs A) {
+// Batched (kB): each atmosphere's own jobs in its z slice; the shared geometry (surface geometry and the
+// re-sampling and sun maps, which depend only on the planet and the grid) once, in slice 0.
+template <bool kB>
+__global__ void __launch_bounds__(kBlock) prologue_kernel(const DevParams P0, const DevParams* __restrict__ PB,
+                                                          const PrologueArgs A) {
+  const DevParams& P = params_of<kB>(P0, PB);
   const unsigned b = blockIdx.x;
+  const bool shared = !kB || blockIdx.z == 0;
   if (b < A.first[1])
-    direction_geometry_body(P, A.j.sdirs, A.j.sphere_n, (DirGeom*)A.j.geom, A.j.wc, b - A.first[0]);
+    direction_geometry_body(P, A.j.sdirs, A.j.sphere_n, atm_ptr<kB>((DirGeom*)A.j.geom, P), A.j.wc, b - A.first[0]);
   else if (b < A.first[2])
-    transmittance_body(P, A.j.T, A.j.wc, b - A.first[1]);
+    transmittance_body(P, atm_ptr<kB>(A.j.T, P), A.j.wc, b - A.first[1]);
   else if (b < A.first[3])
-    surface_radiance_base_body(P, A.j.E0, A.j.wc, b - A.first[2]);
-  else if (b < A.first[4])
-    surface_geometry_body(P, A.j.hdirs, A.j.half_n, b - A.first[3]);
-  else if (b < A.first[5])
+    surface_radiance_base_body(P, atm_ptr<kB>(A.j.E0, P), A.j.wc, b - A.first[2]);
+  else if (b < A.first[4]) {
+    if (shared) surface_geometry_body(P, A.j.hdirs, A.j.half_n, b - A.first[3]);
+  } else if (b < A.first[5])
     sun_depth_body(P, b - A.first[4]);
   else if (b < A.first[6])
     ray_geometry_body(P, A.j.wc, b - A.first[5]);
-  else
+  else if (shared)
     resample_map_body(P, b - A.first[6]);
 }
 
@@ -1585,10 +1632,14 @@ __device__ __forceinline__ void resample_entry(const DevParams& P, int space, co
   }
 }
 
-__global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P, int space, const float4* __restrict__ a,
+template <bool kB>
+__global__ void __launch_bounds__(kBlock) resample_kernel(const DevParams P0, const DevParams* __restrict__ PB,
+                                                          int space, const float4* __restrict__ a,
                                                           const float4* __restrict__ b, float4* __restrict__ out4,
                                                           float* __restrict__ out_rgb, int tiled) {
-  resample_entry(P, space, a, b, out4, out_rgb, tiled, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  const DevParams& P = params_of<kB>(P0, PB);
+  resample_entry(P, space, atm_ptr<kB>(a, P), atm_ptr<kB>(b, P), atm_ptr<kB>(out4, P), atm_ptr<kB>(out_rgb, P), tiled,
+                 (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Several independent re-samplings in one launch: job k takes workgroups first[k] .. first[k + 1] - 1.
@@ -1598,12 +1649,16 @@ struct ResampleJobs {
   int count;
 };
 
-__global__ void __launch_bounds__(kBlock) resample_jobs_kernel(const DevParams P, const ResampleJobs J) {
+template <bool kB>
+__global__ void __launch_bounds__(kBlock) resample_jobs_kernel(const DevParams P0, const DevParams* __restrict__ PB,
+                                                               const ResampleJobs J) {
+  const DevParams& P = params_of<kB>(P0, PB);
   int k = 0;
   while (k + 1 < J.count && blockIdx.x >= J.first[k + 1]) ++k;
   const ResampleJob& j = J.job[k];
-  resample_entry(P, j.space, j.a, j.b, j.out4, j.out_rgb, j.tiled,
-                 (long long)(blockIdx.x - J.first[k]) * blockDim.x + threadIdx.x, j.c);
+  resample_entry(P, j.space, atm_ptr<kB>(j.a, P), atm_ptr<kB>(j.b, P), atm_ptr<kB>(j.out4, P),
+                 atm_ptr<kB>(j.out_rgb, P), j.tiled, (long long)(blockIdx.x - J.first[k]) * blockDim.x + threadIdx.x,
+                 atm_ptr<kB>(j.c, P));
 }
 
 __global__ void pack_rgb_kernel(const float4* __restrict__ in, float* __restrict__ out, long long count) {
@@ -1702,18 +1757,21 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + kBlock - 1) / k
 }  // namespace
 
 hipError_t launch_transmittance(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream) {
+  if (P.batch > 0) return hipErrorInvalidValue;  // batches run these jobs in launch_prologue
   long long n = (long long)P.nT[0] * P.nT[1];
   hipLaunchKernelGGL(transmittance_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, out, wc);
   return hipGetLastError();
 }
 
 hipError_t launch_surface_radiance_base(const DevParams& P, float4* out, WorkCounters* wc, hipStream_t stream) {
+  if (P.batch > 0) return hipErrorInvalidValue;  // batches run these jobs in launch_prologue
   long long n = (long long)P.nE[0] * P.nE[1];
   hipLaunchKernelGGL(surface_radiance_base_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, out, wc);
   return hipGetLastError();
 }
 
 hipError_t launch_ray_geometry(const DevParams& P, WorkCounters* wc, hipStream_t stream) {
+  if (P.batch > 0) return hipErrorInvalidValue;  // batches run these jobs in launch_prologue
   if (!P.ray_geo) return hipErrorInvalidValue;
   hipLaunchKernelGGL(ray_geometry_kernel, dim3(blocks_for((long long)ray_geo_samples(P))), dim3(kBlock), 0, stream, P,
                      wc);
@@ -1723,6 +1781,8 @@ hipError_t launch_ray_geometry(const DevParams& P, WorkCounters* wc, hipStream_t
 hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* mie, WorkCounters* wc,
                                  hipStream_t stream, bool sun_table_built) {
   unsigned rays = (unsigned)(P.nS[0] * P.nS[1]);
+  // a batch runs the table path with the tables built by its prologue
+  if (P.batch > 0 && (P.sun_rows <= 0 || !sun_table_built)) return hipErrorInvalidValue;
   if (P.sun_rows > 0) {
     if (!P.sun_depth || !P.ray_geo || !P.sun_ss || P.sun_rows < 2 || P.sun_cols < 2) return hipErrorInvalidValue;
     if (!sun_table_built)
@@ -1732,8 +1792,12 @@ hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* m
     if (e != hipSuccess) return e;
     const RayGeoView V = ray_geo_view(P);
     const size_t shmem = (size_t)P.ray_steps * (sizeof(double2) + 2 * sizeof(float4) + sizeof(float2));
-    hipLaunchKernelGGL(single_scatter_table_kernel, dim3(rays), dim3(kSSTBlock), shmem, stream, P, V.th, V.T, V.ss,
-                       V.row, rayleigh, mie, wc);
+    if (P.batch > 0)
+      hipLaunchKernelGGL(single_scatter_table_kernel<true>, dim3(rays, 1, P.batch), dim3(kSSTBlock), shmem, stream, P,
+                         P.batch_params, V.th, V.T, V.ss, V.row, rayleigh, mie, wc);
+    else
+      hipLaunchKernelGGL(single_scatter_table_kernel<false>, dim3(rays), dim3(kSSTBlock), shmem, stream, P,
+                         P.batch_params, V.th, V.T, V.ss, V.row, rayleigh, mie, wc);
     return hipGetLastError();
   }
   unsigned groups = (unsigned)((P.nS[2] * P.nS[3] + kSSWave - 1) / kSSWave);
@@ -1746,6 +1810,7 @@ hipError_t launch_single_scatter(const DevParams& P, float4* rayleigh, float4* m
 
 hipError_t launch_direction_geometry(const DevParams& P, const Dir* dirs, int ndirs, void* geom, WorkCounters* wc,
                                     hipStream_t stream) {
+  if (P.batch > 0) return hipErrorInvalidValue;  // batches run these jobs in launch_prologue
   long long n = (long long)P.nS[0] * ndirs;
   hipLaunchKernelGGL(direction_geometry_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, dirs, ndirs,
                      (DirGeom*)geom, wc);
@@ -1759,8 +1824,13 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
                                 WorkCounters* wc, hipStream_t stream, const SrJob* sr) {
   if (!P.sun_ps) return hipErrorInvalidValue;
   unsigned combos = (unsigned)(P.nS[0] * ndirs);
-  hipLaunchKernelGGL(point_scatter_combine_kernel, dim3(combos), dim3(kBlock), 0, stream, P, (const DirGeom*)geom, ndirs,
-                     dS, mie, slab, mie ? slab_mie : nullptr);
+  const unsigned nz = P.batch > 0 ? (unsigned)P.batch : 1u;
+  if (P.batch > 0)
+    hipLaunchKernelGGL(point_scatter_combine_kernel<true>, dim3(combos, 1, nz), dim3(kBlock), 0, stream, P,
+                       P.batch_params, (const DirGeom*)geom, ndirs, dS, mie, slab, mie ? slab_mie : nullptr);
+  else
+    hipLaunchKernelGGL(point_scatter_combine_kernel<false>, dim3(combos), dim3(kBlock), 0, stream, P, P.batch_params,
+                       (const DirGeom*)geom, ndirs, dS, mie, slab, mie ? slab_mie : nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t e_bytes = sizeof(float4) * (size_t)P.nE[1];  // dE height row 0, the ground
@@ -1777,18 +1847,35 @@ hipError_t launch_point_scatter(const DevParams& P, const Dir* dirs, int ndirs, 
   }
   const unsigned blocks = (unsigned)(P.nS[0] * P.nS[1]) + extra;
   const DirGeom* g = (const DirGeom*)geom;
-  if (mie && e_lds)
-    hipLaunchKernelGGL((point_scatter_kernel<true, true>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
-                       slab, slab_mie, dE, out, job, wc);
-  else if (mie)
-    hipLaunchKernelGGL((point_scatter_kernel<true, false>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
-                       slab, slab_mie, dE, out, job, wc);
-  else if (e_lds)
-    hipLaunchKernelGGL((point_scatter_kernel<false, true>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
-                       slab, nullptr, dE, out, job, wc);
-  else
-    hipLaunchKernelGGL((point_scatter_kernel<false, false>), dim3(blocks), dim3(kBlock), shmem, stream, P, dirs, g, ndirs,
-                       slab, nullptr, dE, out, job, wc);
+  const dim3 grid(blocks, 1, nz);
+  const DevParams* PB = P.batch_params;
+  if (P.batch > 0) {
+    if (mie && e_lds)
+      hipLaunchKernelGGL((point_scatter_kernel<true, true, true>), grid, dim3(kBlock), shmem, stream, P, PB, dirs, g,
+                         ndirs, slab, slab_mie, dE, out, job, wc);
+    else if (mie)
+      hipLaunchKernelGGL((point_scatter_kernel<true, false, true>), grid, dim3(kBlock), shmem, stream, P, PB, dirs, g,
+                         ndirs, slab, slab_mie, dE, out, job, wc);
+    else if (e_lds)
+      hipLaunchKernelGGL((point_scatter_kernel<false, true, true>), grid, dim3(kBlock), shmem, stream, P, PB, dirs, g,
+                         ndirs, slab, nullptr, dE, out, job, wc);
+    else
+      hipLaunchKernelGGL((point_scatter_kernel<false, false, true>), grid, dim3(kBlock), shmem, stream, P, PB, dirs, g,
+                         ndirs, slab, nullptr, dE, out, job, wc);
+  } else {
+    if (mie && e_lds)
+      hipLaunchKernelGGL((point_scatter_kernel<true, true, false>), grid, dim3(kBlock), shmem, stream, P, PB, dirs, g,
+                         ndirs, slab, slab_mie, dE, out, job, wc);
+    else if (mie)
+      hipLaunchKernelGGL((point_scatter_kernel<true, false, false>), grid, dim3(kBlock), shmem, stream, P, PB, dirs, g,
+                         ndirs, slab, slab_mie, dE, out, job, wc);
+    else if (e_lds)
+      hipLaunchKernelGGL((point_scatter_kernel<false, true, false>), grid, dim3(kBlock), shmem, stream, P, PB, dirs, g,
+                         ndirs, slab, nullptr, dE, out, job, wc);
+    else
+      hipLaunchKernelGGL((point_scatter_kernel<false, false, false>), grid, dim3(kBlock), shmem, stream, P, PB, dirs,
+                         g, ndirs, slab, nullptr, dE, out, job, wc);
+  }
   return hipGetLastError();
 }
 
@@ -1817,21 +1904,35 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   const size_t shmem =
       2 * sizeof(float4) * (size_t)Q * nla + (size_t)P.ray_steps * (sizeof(int4) + sizeof(float4) + sizeof(int));
   const RayGeoView V = ray_geo_view(P);
-  if (nla <= kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<1>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc, Q,
-                       wc);
-  else if (nla <= 2 * kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<2>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc, Q,
-                       wc);
-  else if (nla <= 4 * kRSHalf)
-    hipLaunchKernelGGL(ray_scatter_kernel<4>, dim3(rays), dim3(kRSBlock), shmem, stream, P, dJ, V.T, V.rs, out, acc, Q,
-                       wc);
-  else
-    return hipErrorInvalidValue;
+  const DevParams* PB = P.batch_params;
+  if (nla > 4 * kRSHalf) return hipErrorInvalidValue;
+  if (P.batch > 0) {
+    const dim3 grid(rays, 1, (unsigned)P.batch);
+    if (nla <= kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<1, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs, out,
+                         acc, Q, wc);
+    else if (nla <= 2 * kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<2, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs, out,
+                         acc, Q, wc);
+    else
+      hipLaunchKernelGGL((ray_scatter_kernel<4, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs, out,
+                         acc, Q, wc);
+  } else {
+    if (nla <= kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<1, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
+                         out, acc, Q, wc);
+    else if (nla <= 2 * kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<2, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
+                         out, acc, Q, wc);
+    else
+      hipLaunchKernelGGL((ray_scatter_kernel<4, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
+                         out, acc, Q, wc);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_surface_geometry(const DevParams& P, const Dir* dirs, int ndirs, hipStream_t stream) {
+  if (P.batch > 0) return hipErrorInvalidValue;  // batches run these jobs in launch_prologue
   if (!P.sgeo) return hipErrorInvalidValue;
   hipLaunchKernelGGL(surface_geometry_kernel, dim3(blocks_for((long long)P.nE[0] * ndirs)), dim3(kBlock), 0, stream, P,
                      dirs, ndirs);
@@ -1847,11 +1948,23 @@ hipError_t launch_surface_radiance(const DevParams& P, int ndirs, const float4* 
   if (!P.sgeo) return hipErrorInvalidValue;
   unsigned entries = (unsigned)(P.nE[0] * P.nE[1]);
   const size_t fold = sr_fold_bytes(P, mie != nullptr);
-  if (fold <= 65536)
-    hipLaunchKernelGGL(surface_radiance_kernel<true>, dim3(entries), dim3(kBlock), fold, stream, P, ndirs, dS, mie, out,
-                       wc);
-  else
-    hipLaunchKernelGGL(surface_radiance_kernel<false>, dim3(entries), dim3(kBlock), 0, stream, P, ndirs, dS, mie, out, wc);
+  const dim3 grid(entries, 1, P.batch > 0 ? (unsigned)P.batch : 1u);
+  const DevParams* PB = P.batch_params;
+  if (P.batch > 0) {
+    if (fold <= 65536)
+      hipLaunchKernelGGL((surface_radiance_kernel<true, true>), grid, dim3(kBlock), fold, stream, P, PB, ndirs, dS, mie,
+                         out, wc);
+    else
+      hipLaunchKernelGGL((surface_radiance_kernel<false, true>), grid, dim3(kBlock), 0, stream, P, PB, ndirs, dS, mie,
+                         out, wc);
+  } else {
+    if (fold <= 65536)
+      hipLaunchKernelGGL((surface_radiance_kernel<true, false>), grid, dim3(kBlock), fold, stream, P, PB, ndirs, dS, mie,
+                         out, wc);
+    else
+      hipLaunchKernelGGL((surface_radiance_kernel<false, false>), grid, dim3(kBlock), 0, stream, P, PB, ndirs, dS, mie,
+                         out, wc);
+  }
   return hipGetLastError();
 }
 
@@ -1869,11 +1982,16 @@ hipError_t launch_prologue(const DevParams& P, const PrologueJobs& J, hipStream_
                                (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3]};
   A.first[0] = 0;
   for (int k = 0; k < 7; ++k) A.first[k + 1] = A.first[k] + blocks_for(counts[k]);
-  hipLaunchKernelGGL(prologue_kernel, dim3(A.first[7]), dim3(kBlock), 0, stream, P, A);
+  if (P.batch > 0)
+    hipLaunchKernelGGL(prologue_kernel<true>, dim3(A.first[7], 1, P.batch), dim3(kBlock), 0, stream, P,
+                       P.batch_params, A);
+  else
+    hipLaunchKernelGGL(prologue_kernel<false>, dim3(A.first[7]), dim3(kBlock), 0, stream, P, P.batch_params, A);
   return hipGetLastError();
 }
 
 hipError_t launch_resample_map(const DevParams& P, hipStream_t stream) {
+  if (P.batch > 0) return hipErrorInvalidValue;  // batches run these jobs in launch_prologue
   if (!P.rmap) return hipErrorInvalidValue;
   const long long n = (long long)P.nS[0] * P.nS[1] * P.nS[2] * P.nS[3];
   hipLaunchKernelGGL(resample_map_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P);
@@ -1899,15 +2017,24 @@ hipError_t launch_resamples(const DevParams& P, const ResampleJob* jobs, int cou
       J.first[k + 1] = J.first[k] + blocks_for(resample_count(P, jobs[k].space));
     }
   }
-  hipLaunchKernelGGL(resample_jobs_kernel, dim3(J.first[count]), dim3(kBlock), 0, stream, P, J);
+  if (P.batch > 0)
+    hipLaunchKernelGGL(resample_jobs_kernel<true>, dim3(J.first[count], 1, P.batch), dim3(kBlock), 0, stream, P,
+                       P.batch_params, J);
+  else
+    hipLaunchKernelGGL(resample_jobs_kernel<false>, dim3(J.first[count]), dim3(kBlock), 0, stream, P, P.batch_params,
+                       J);
   return hipGetLastError();
 }
 
 hipError_t launch_resample(const DevParams& P, int space, const float4* a, const float4* b, float4* out4,
                            float* out_rgb, int tiled, hipStream_t stream) {
   const long long n = resample_count(P, space);
-  hipLaunchKernelGGL(resample_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, space, a, b, out4, out_rgb,
-                     tiled);
+  if (P.batch > 0)
+    hipLaunchKernelGGL(resample_kernel<true>, dim3(blocks_for(n), 1, P.batch), dim3(kBlock), 0, stream, P,
+                       P.batch_params, space, a, b, out4, out_rgb, tiled);
+  else
+    hipLaunchKernelGGL(resample_kernel<false>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, P, P.batch_params, space,
+                       a, b, out4, out_rgb, tiled);
   return hipGetLastError();
 }
 
