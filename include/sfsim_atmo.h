@@ -132,6 +132,20 @@ int sfsim_atmo_resample_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, i
 int sfsim_atmo_generate_luts(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* transmittance,
                              float* surface_radiance, float* ray_scatter, float* mie_strength);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): sfsim_atmo_generate_luts with the four
+ * final tables written to the caller's DEVICE buffers (layouts and sizes as generate_luts, none NULL) and
+ * every intermediate table in one static device array of SFSIM_ATMO_NOALLOC_SCRATCH_BYTES, which each GPU
+ * the library runs on holds until the process ends.  The call allocates and frees no device memory; it
+ * copies the quadrature directions and the ray order from the host and returns after the stream has drained.  A
+ * workload whose intermediate tables do not fit the array is rejected with SFSIM_ATMO_ERR_ARGUMENT (the
+ * reference's production workload takes 265 MiB of it, the 32x64x32x8 5-order workload 138 MiB).  All
+ * calls on one GPU share its array, which keeps the last call's contents: calls that overlap on one GPU
+ * (several contexts or threads) race and must be serialised by the caller. */
+#define SFSIM_ATMO_NOALLOC_SCRATCH_BYTES ((size_t)288 << 20)
+int sfsim_atmo_generate_luts_noalloc(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* transmittance,
+                                     float* surface_radiance, float* ray_scatter, float* mie_strength);
+
 /* Device-resident build for callers that keep the tables on the GPU (and for timing): runs the
  * whole generate-atmosphere-luts pipeline into context-owned device buffers without host copies;
  * sfsim_atmo_fetch then copies one final table out (which: 0 = transmittance, 1 = surface

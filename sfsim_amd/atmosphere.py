@@ -31,7 +31,7 @@ EXPORTS = (
     "sfsim_atmo_surrogate_weights", "sfsim_atmo_surrogate_set_weights", "sfsim_atmo_surrogate_eval",
     "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_build_from_sums", "sfsim_atmo_order_sums",
     "sfsim_atmo_surrogate_predict", "sfsim_atmo_set_sun_table", "sfsim_atmo_get_sun_table",
-    "sfsim_atmo_build_batch", "sfsim_atmo_fetch_batch",
+    "sfsim_atmo_generate_luts_noalloc", "sfsim_atmo_build_batch", "sfsim_atmo_fetch_batch",
     "sfsim_atmo_standard_atmosphere", "sfsim_atmo_speed_of_sound",
 )
 
@@ -141,6 +141,7 @@ def lib():
         L.sfsim_atmo_surface_radiance_table.argtypes = [vp, pp, fp, fp, fp]
         L.sfsim_atmo_resample_table.argtypes = [vp, pp, ctypes.c_int, fp, fp, fp]
         L.sfsim_atmo_generate_luts.argtypes = [vp, pp, fp, fp, fp, fp]
+        L.sfsim_atmo_generate_luts_noalloc.argtypes = [vp, pp, vp, vp, vp, vp]
         L.sfsim_atmo_build.argtypes = [vp, pp]
         L.sfsim_atmo_build_timed.argtypes = [vp, pp]
         L.sfsim_atmo_fetch.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
@@ -331,6 +332,14 @@ class Atmosphere:
         M = np.zeros_like(S)
         self._check(lib().sfsim_atmo_generate_luts(self._ctx, ctypes.byref(P), _fp(T), _fp(E), _fp(S), _fp(M)))
         return {"transmittance": T, "surface-radiance": E, "ray-scatter": S, "mie-strength": M}
+
+    def generate_luts_noalloc(self, P, outputs):
+        """sfsim_atmo_generate_luts_noalloc: the four tables of generate_atmosphere_luts written to
+        caller-owned device buffers (``outputs``: their four device addresses, in the order transmittance,
+        surface radiance, ray scatter, Mie strength); the intermediate tables live in the library's
+        static device scratch array, so the call allocates no device memory."""
+        ptrs = [ctypes.c_void_p(int(a)) if a else None for a in outputs]
+        self._check(lib().sfsim_atmo_generate_luts_noalloc(self._ctx, ctypes.byref(P), *ptrs))
 
     def build(self, P):
         """sfsim_atmo_build: graph-replayed device-resident build."""

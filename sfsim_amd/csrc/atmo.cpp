@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -53,12 +54,45 @@ struct DeviceBuffer {
 // 1024 x 1024: 4.6e-5, 6% slower; fewer columns cost more accuracy than fewer rows: 768 x 640 9.8e-5).
 constexpr int kSunRows = 640, kSunCols = 768;
 
+// Device memory of sfsim_atmo_generate_luts_noalloc: every intermediate table of a build lives here, so
+// that entry point allocates nothing.  Each GPU the library runs on holds its own copy of the array, at
+// its own address, from the module's load until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: kernels get the address that hipGetSymbolAddress returned for
+// the current device (scratch_address).
+static __device__ __align__(256) unsigned char scratch[SFSIM_ATMO_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current: looked up once per device.
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+}  // namespace
+
 struct sfsim_atmo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string error;
-  // set on the stack context of a batched build: every buffer is pre-assigned in the batch's memory, and a
-  // request that does not fit is an error instead of a hipMalloc
+  // device address of the scratch array on this context's device (sfsim_atmo_generate_luts_noalloc)
+  void* scratch = nullptr;
+  // set on the stack contexts of batched builds and of sfsim_atmo_generate_luts_noalloc: every buffer is
+  // pre-assigned (in the batch's memory or the scratch array), and a request that does not fit is an error
+  // instead of a hipMalloc
   bool no_alloc = false;
   // tables (float4 per entry)
   DeviceBuffer T, dE[2], Eacc[2], R1, M1, dJ, dS[2], Sacc[2];
@@ -1032,6 +1066,13 @@ size_t layout_atmosphere(sfsim_atmo_ctx* c, const sfsim_atmo_params* p, char* ba
   return k.off;
 }
 
+// Every buffer that prepare_pipeline asks for (the outputs aside), in one piece of device memory at
+// `base`; returns the bytes they take (base == nullptr: the size only).
+size_t noalloc_layout(sfsim_atmo_ctx* c, const sfsim_atmo_params* p, char* base) {
+  const size_t shared = layout_shared(c, p, base);
+  return shared + layout_atmosphere(c, p, base ? base + shared : nullptr, false);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1052,6 +1093,14 @@ int sfsim_atmo_create(int device, sfsim_atmo_ctx** out) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return SFSIM_ATMO_ERR_DEVICE;
+  }
+  // the scratch array's address on this device (the device is current); without it only
+  // sfsim_atmo_generate_luts_noalloc fails, with the reason
+  const hipError_t se = scratch_address(device, &ctx->scratch);
+  if (se != hipSuccess) {
+    ctx->scratch = nullptr;
+    ctx->error = std::string("scratch array unavailable: ") + hipGetErrorString(se);
+    (void)hipGetLastError();
   }
   *out = ctx;
   return 0;
@@ -1259,6 +1308,47 @@ int sfsim_atmo_generate_luts(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, fl
   for (int i = 0; i < 4; ++i)
     if (outs[i]) TRY(sfsim_atmo_fetch(ctx, i, outs[i], ctx->out_count[i]));
   return 0;
+}
+
+int sfsim_atmo_generate_luts_noalloc(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* transmittance,
+                                     float* surface_radiance, float* ray_scatter, float* mie_strength) {
+  TRY(activate(ctx));
+  TRY(validate(ctx, p));
+  if (!transmittance || !surface_radiance || !ray_scatter || !mie_strength)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "the four output device buffers must not be NULL");
+  if (p->iterations > 0) {
+    TRY(validate_sphere(ctx, p->sphere_steps));
+    TRY(validate_half_sphere(ctx, p->ray_steps));
+  }
+  if (!ctx->scratch)
+    return fail(ctx, SFSIM_ATMO_ERR_DEVICE, ctx->error.empty() ? "scratch array unavailable" : ctx->error);
+  // a context on the stack whose buffers all lie in the scratch array: the driver's ensure() calls find
+  // them in place (and fail instead of allocating if one were missing)
+  sfsim_atmo_ctx sh;
+  sh.device = ctx->device;
+  sh.stream = ctx->stream;
+  sh.no_alloc = true;
+  sh.sun_rows = ctx->sun_rows;
+  sh.sun_cols = ctx->sun_cols;
+  const size_t need = noalloc_layout(&sh, p, nullptr);
+  if (need > SFSIM_ATMO_NOALLOC_SCRATCH_BYTES) {
+    char msg[160];
+    std::snprintf(msg, sizeof(msg), "the workload needs %zu bytes of scratch, more than the %zu of the no-alloc entry point",
+                  need, (size_t)SFSIM_ATMO_NOALLOC_SCRATCH_BYTES);
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, msg);
+  }
+  noalloc_layout(&sh, p, (char*)ctx->scratch);
+  const size_t nT = (size_t)count2(p->transmittance_shape), nE = (size_t)count2(p->surface_radiance_shape);
+  const size_t nS = (size_t)count4(p->scatter_shape);
+  const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
+  float* outs[4] = {transmittance, surface_radiance, ray_scatter, mie_strength};
+  for (int i = 0; i < 4; ++i) {
+    sh.out[i].ptr = outs[i];
+    sh.out[i].bytes = outc[i] * sizeof(float);
+  }
+  const int rc = run_pipeline(&sh, p, nullptr, false);
+  if (rc) ctx->error = sh.error;
+  return rc;
 }
 
 int sfsim_atmo_build_batch(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* ps, int count) {
