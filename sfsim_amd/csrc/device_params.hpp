@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <type_traits>
 
 #include "geometry.hpp"
 
@@ -74,21 +75,42 @@ struct DevParams {
   const DevParams* batch_params;
 };
 
-// The parameters of the launch's atmosphere and its own copy of a per-atmosphere table.
+// The parameters of the launch's atmosphere.  Batched: the scalars from the device array, the table
+// pointers from the kernel's own argument (a global pointer to the compiler, so that loads through it stay
+// global rather than flat loads), offset to the atmosphere's block where the table is per atmosphere (the
+// ray geometry and the sun table; the re-sampling and sun maps, the half-sphere geometry and the ray order
+// are shared).  Single atmosphere: the kernel argument itself.
 template <bool kB>
-__device__ __forceinline__ const DevParams& params_of(const DevParams& P0, const DevParams* __restrict__ PB) {
-  if constexpr (kB)
-    return PB[blockIdx.z];
-  else
-    return P0;
+__device__ __forceinline__ decltype(auto) params_of(const DevParams& P0, const DevParams* __restrict__ PB) {
+  if constexpr (kB) {
+    DevParams P = PB[blockIdx.z];
+    const size_t off = (size_t)blockIdx.z * P0.atm_stride;
+    P.sun_depth = P0.sun_depth ? reinterpret_cast<float2*>(reinterpret_cast<char*>(P0.sun_depth) + off) : nullptr;
+    P.ray_geo = P0.ray_geo ? P0.ray_geo + off : nullptr;
+    P.rmap = P0.rmap;
+    P.sun_ps = P0.sun_ps;
+    P.sun_rs = P0.sun_rs;
+    P.sun_ss = P0.sun_ss;
+    P.sgeo = P0.sgeo;
+    P.ray_order = P0.ray_order;
+    return P;
+  } else {
+    return (P0);
+  }
 }
 
+// This atmosphere's copy of a per-atmosphere table.
+
+// (Byte arithmetic on a char pointer, not an integer round trip, so that the compiler still sees a global
+// pointer and emits global rather than flat loads.)
 template <bool kB, class T>
 __device__ __forceinline__ T* atm_ptr(T* p, const DevParams& P) {
-  if constexpr (kB)
-    return p ? (T*)((uintptr_t)p + (uintptr_t)blockIdx.z * P.atm_stride) : p;
-  else
+  if constexpr (kB) {
+    using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return p ? reinterpret_cast<T*>(reinterpret_cast<Byte*>(p) + (size_t)blockIdx.z * P.atm_stride) : p;
+  } else {
     return p;
+  }
 }
 
 __host__ __device__ inline size_t resample_map_bytes(const DevParams& P) {

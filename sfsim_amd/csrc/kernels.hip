@@ -849,18 +849,12 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs, float4* __restrict__ out,
     const RsAccum accum_arg, int Q, WorkCounters* wc) {
   const DevParams& P = params_of<kB>(P0, PB);
-  RsAccum accum = accum_arg;
-  if constexpr (kB) {  // this atmosphere's tables and ray geometry
+  if constexpr (kB) {  // this atmosphere's tables and ray geometry (the accumulation tables where they are used)
     const RayGeoView V = ray_geo_view(P);
     geo_T = V.T;
     geo_rs = V.rs;
     dJ = atm_ptr<kB>(dJ, P);
     out = atm_ptr<kB>(out, P);
-    accum.S_in = atm_ptr<kB>(accum.S_in, P);
-    accum.S_out = atm_ptr<kB>(accum.S_out, P);
-    accum.dE = atm_ptr<kB>(accum.dE, P);
-    accum.E_in = atm_ptr<kB>(accum.E_in, P);
-    accum.E_out = atm_ptr<kB>(accum.E_out, P);
   }
   // dynamic LDS: 2 x Q x (NL x NA) slab entries (chunk c in half c % 2), then per sample the slab
   // offsets (in units of NL x NA), their bilinear weights and the fold slots whose node differs from the
@@ -874,9 +868,12 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
   int* s_mask = (int*)(s_wt + n);
   if ((int)blockIdx.x >= NH * NE) {  // the extra workgroup: E <- tab(dE + E) (atmosphere_lut.clj:95-96)
     const long long nE = (long long)P.nE[0] * P.nE[1];
+    const float4* dE = atm_ptr<kB>(accum_arg.dE, P);
+    const float4* E_in = atm_ptr<kB>(accum_arg.E_in, P);
+    float4* E_out = atm_ptr<kB>(accum_arg.E_out, P);
     for (long long i = threadIdx.x; i < nE; i += blockDim.x) {
-      const float3 v = resample2_value(P, 3, accum.dE, accum.E_in, i);
-      accum.E_out[i] = make_float4(v.x, v.y, v.z, 0.f);
+      const float3 v = resample2_value(P, 3, dE, E_in, i);
+      E_out[i] = make_float4(v.x, v.y, v.z, 0.f);
     }
     return;
   }
@@ -983,10 +980,10 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     if (k < NLA) {
       const float4 d = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
       out[base + k] = d;
-      if (accum.S_out) {  // the running sum S + dS of atmosphere_lut.clj:95-97 at this grid point
-        const float3 u =
-            accum.S_resample ? resample4_value_lean(P, accum.S_in, base + k) : ld3(accum.S_in, base + k);
-        accum.S_out[base + k] = make_float4(u.x + d.x, u.y + d.y, u.z + d.z, 0.f);
+      if (accum_arg.S_out) {  // the running sum S + dS of atmosphere_lut.clj:95-97 at this grid point
+        const float4* S_in = atm_ptr<kB>(accum_arg.S_in, P);
+        const float3 u = accum_arg.S_resample ? resample4_value_lean(P, S_in, base + k) : ld3(S_in, base + k);
+        atm_ptr<kB>(accum_arg.S_out, P)[base + k] = make_float4(u.x + d.x, u.y + d.y, u.z + d.z, 0.f);
       }
       lookups += n;
     }
