@@ -631,12 +631,12 @@ __global__ void __launch_bounds__(kSSTBlock) single_scatter_table_kernel(
     const float4* __restrict__ geo_T, const float4* __restrict__ geo_ss, const float2* __restrict__ geo_row,
     float4* __restrict__ R1, float4* __restrict__ M1, WorkCounters* wc) {
   const DevParams& P = params_of<kB>(P0, PB);
-  if constexpr (kB) {  // this atmosphere's tables and ray geometry
-    const RayGeoView V = ray_geo_view(P);
-    geo_th = V.th;
-    geo_T = V.T;
-    geo_ss = V.ss;
-    geo_row = V.row;
+  if constexpr (kB) {  // this atmosphere's tables and ray geometry (offsets of the restrict arguments, whose
+                       // loads stay provably unclobbered and uniform: scalar loads)
+    geo_th = atm_ptr<kB>(geo_th, P0);
+    geo_T = atm_ptr<kB>(geo_T, P0);
+    geo_ss = atm_ptr<kB>(geo_ss, P0);
+    geo_row = atm_ptr<kB>(geo_row, P0);
     R1 = atm_ptr<kB>(R1, P);
     M1 = atm_ptr<kB>(M1, P);
   }
@@ -792,7 +792,10 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
                                                  int nt, FoldSlots& f) {
   if (NLA <= nt) {
     const int kk = t;
-    const int kl = min(kk, NLA - 1);  // lanes past the table read a valid entry and write nothing
+    // lanes past the table read the last entry and store the same bits as the last entry's lane to the
+    // same place: no branch around the store, which would give the loop a path that skips the wait for
+    // the slot loads and make the next sample's loads wait for them one by one
+    const int kl = min(kk, NLA - 1);
     for (int q = 0; q < qn; ++q) {
       const int m = __builtin_amdgcn_readfirstlane(s_mask[i0 + q]);
       const float4 w = s_wt[i0 + q];
@@ -804,10 +807,9 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
         if (m & 8) f.v[3] = dJ[__builtin_amdgcn_readfirstlane(o.w) + kl];
       }
       const float4 a = f.v[0], b = f.v[1], c = f.v[2], d = f.v[3];
-      if (kk < NLA)
-        slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
-                                         fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
-                                         fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+      slab[q * NLA + kl] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
+                                       fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
+                                       fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
     }
   } else {
     for (int q = 0; q < qn; ++q) {
@@ -849,10 +851,10 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs, float4* __restrict__ out,
     const RsAccum accum_arg, int Q, WorkCounters* wc) {
   const DevParams& P = params_of<kB>(P0, PB);
-  if constexpr (kB) {  // this atmosphere's tables and ray geometry (the accumulation tables where they are used)
-    const RayGeoView V = ray_geo_view(P);
-    geo_T = V.T;
-    geo_rs = V.rs;
+  if constexpr (kB) {  // this atmosphere's tables and ray geometry (the accumulation tables where they are used);
+                       // offsets of the restrict arguments, whose loads stay provably unclobbered and uniform
+    geo_T = atm_ptr<kB>(geo_T, P0);
+    geo_rs = atm_ptr<kB>(geo_rs, P0);
     dJ = atm_ptr<kB>(dJ, P);
     out = atm_ptr<kB>(out, P);
   }
