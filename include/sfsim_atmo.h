@@ -198,23 +198,27 @@ int sfsim_atmo_surrogate_set_weights(sfsim_atmo_ctx* ctx, const float* in, size_
  * 15 used) also the loss (mean over rows of half the squared error) and its gradient (size() floats). */
 int sfsim_atmo_surrogate_eval(sfsim_atmo_ctx* ctx, const float* inputs, const float* targets, int rows,
                               float* outputs, float* grad, float* loss);
-/* Trains on the 5-order plain builds of `count` atmospheres sharing one scatter shape: `steps` Adam
+/* Trains on the first five orders of the plain builds of `count` atmospheres (iterations >= 5) sharing one
+ * scatter shape: `steps` Adam
  * steps of one batch each, learning rate lr with a linear warm-up over the first 5% of the steps and
  * cosine decay after it (a fresh network's output biases start at the targets' means); losses
  * (nullable) receives one value per step. */
 int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atmospheres, int count, int steps,
                                double lr, unsigned long long seed, float* losses);
-/* Device-resident 5-order build seeded by the network with `applications` (1..4) exact applications
- * of the iteration (outputs as sfsim_atmo_build, read with sfsim_atmo_fetch); residual receives the
- * relative residual, fallback 1 when the plain iteration ran instead. */
+/* Device-resident build of p->iterations (5..32) scattering orders seeded by the network, which predicts
+ * the order sums P_1..P_5: the partial sum of iterations - applications orders (1..4) comes from the network
+ * and `applications` exact applications of the iteration (atmosphere_lut.clj:86-97) follow, so applications
+ * is in iterations-4 .. iterations-1 (outputs as sfsim_atmo_build, read with sfsim_atmo_fetch).  residual
+ * receives the network's own P_5 against the computed one, relative to the ray-scatter table; fallback is 1
+ * when it exceeded the tolerance and the plain iteration ran instead. */
 int sfsim_atmo_build_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int applications, double tolerance,
                             double* residual, int* fallback);
 /* The same with the order sums P_1..P_5 supplied by the caller (5 RGB scatter tables one after the
  * other, e.g. those of a neighbouring atmosphere in a parameter sweep) instead of the network's. */
 int sfsim_atmo_build_from_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int applications, const float* sums,
                                double tolerance, double* residual, int* fallback);
-/* The order sums P_1..P_5 of the plain iteration, and the network's prediction of them: sums holds
- * 5 scatter tables (RGB) one after the other. */
+/* The order sums P_1..P_5 of the plain iteration (iterations >= 5; only the first five orders run), and
+ * the network's prediction of them: sums holds 5 scatter tables (RGB) one after the other. */
 int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* sums);
 int sfsim_atmo_surrogate_predict(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* sums);
 

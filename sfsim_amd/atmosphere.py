@@ -456,15 +456,17 @@ ORDERS = 5
 
 
 def order_sums(ctx, P):
-    """P_k = dS_1 + ... + dS_k (k = 1..5) of the plain iteration: array [5, *scatter_shape, 3]."""
+    """P_k = dS_1 + ... + dS_k (k = 1..5) of the plain iteration (P.iterations >= 5; the first five
+    orders run): array [5, *scatter_shape, 3]."""
     out = np.zeros((ORDERS,) + tuple(P.scatter_shape) + (3,), np.float32)
     ctx._check(lib().sfsim_atmo_order_sums(ctx._ctx, ctypes.byref(P), _fp(out)))
     return out
 
 
 def build_seeded(ctx, P, applications=1, tolerance=1e-3):
-    """sfsim_atmo_build_seeded: the 5-order set from the network's partial sum of 5 - applications
-    orders and `applications` exact steps of the iteration; returns (residual, fell_back)."""
+    """sfsim_atmo_build_seeded: the set of P.iterations (5..32) orders from the network's partial sum of
+    P.iterations - applications orders (1..4) and `applications` exact steps of the iteration; returns
+    (residual, fell_back)."""
     r = ctypes.c_double()
     f = ctypes.c_int()
     ctx._check(lib().sfsim_atmo_build_seeded(ctx._ctx, ctypes.byref(P), applications, tolerance, ctypes.byref(r),

@@ -119,7 +119,7 @@ struct sfsim_atmo_ctx {
   DeviceBuffer counters;
   // surrogate: network state, training set (codes, targets), order sums, predictions, residual scratch
   Surrogate sg;
-  DeviceBuffer codes, targets, sumS, cap[kMlpOrders], pred[kMlpOrders], resid;
+  DeviceBuffer codes, targets, sumS, cap[kMlpOrders], pred[kMaxSeededOrders], resid;
   double last_residual = 0.0;
   // graph of the last build() (replayed while parameters and buffers stay the same)
   hipGraphExec_t graph_exec = nullptr;
@@ -616,7 +616,6 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
   (void)nE;
   auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
   const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
-  const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
 
   TRY(enqueue_prologue(ctx, p, P, wc, timer, merged));
   // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
@@ -757,19 +756,23 @@ AtmosphereCode code_of(const sfsim_atmo_params* p) {
 // the linear recurrence x_k = L x_{k-1} on x_k = (dS_k, dE_k): dS_k = S[J[dS_{k-1}, dE_{k-1}]],
 // dE_k = E[dS_{k-1}], x_0 = (dS_0, E0).  Its partial sums X_j = x_0 + ... + x_j obey
 // X_{j+1} = x_0 + L X_j, and with P_j = dS_1 + ... + dS_j the sum X_j is
-// (dS_0 + P_j, E0 + E[dS_0 + P_{j-1}]).  Seeding X_{5-m} with the network's P_{5-m}, P_{4-m} and applying
-// the recurrence m times gives X_5 with the network's error multiplied by L^m (a contraction);
-// m = `applications` in 1..4.  The network's own P_5 against the computed sum is the residual; above
+// (dS_0 + P_j, E0 + E[dS_0 + P_{j-1}]).  For N iterations, seeding X_{N-m} with the network's P_{N-m},
+// P_{N-m-1} and applying the recurrence m times gives X_N with the network's error multiplied by L^m (a
+// contraction); the network predicts P_1..P_5, so m = `applications` is in N-4..N-1 (the seeded order is
+// at most 4 and P_5 is computed).  The network's own P_5 against the computed sum is the residual; above
 // `tol` the plain iteration runs instead.  The final ray-scatter table then follows the plain
 // iteration's accumulation, re-sampled order by order, on P_1..P_j of the seed and the computed
-// P_{j+1}..P_5.
+// P_{j+1}..P_N.
 
 // Validation and buffers of a seeded build (everything that allocates or copies from the host but the
 // caller's order sums).
 int prepare_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, const float* host_sums) {
   TRY(validate(ctx, p));
-  if (p->iterations != kMlpOrders) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "the surrogate stands in for a 5-order set");
-  if (m < 1 || m > kMlpOrders - 1) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "applications must be in 1..4");
+  const int N = p->iterations;
+  if (N < kMlpOrders || N > kMaxSeededOrders)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "a seeded build has 5..32 scattering orders");
+  if (m < N - (kMlpOrders - 1) || m > N - 1)
+    return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "applications must be in iterations-4 .. iterations-1");
   if (!host_sums && !ctx->sg.trained) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "surrogate is not trained");
   if (host_sums && !finite_floats(host_sums, (size_t)kMlpOrders * count4(p->scatter_shape) * 3))
     return fail(ctx, SFSIM_ATMO_ERR_NONFINITE, "order sums hold NaN/Inf");
@@ -792,7 +795,7 @@ int prepare_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, int m, const
   TRY(ensure(ctx, ctx->R1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->M1, nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->dJ, nS * sizeof(float4)));
-  for (int i = 0; i < kMlpOrders; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
+  for (int i = 0; i < N; ++i) TRY(ensure(ctx, ctx->pred[i], nS * sizeof(float4)));
   TRY(ensure(ctx, ctx->resid, residual_scratch_floats((long long)nS) * sizeof(float)));
   if (host_sums) TRY(ensure(ctx, ctx->stage_rgb, nS * 3 * sizeof(float)));
   const size_t outc[4] = {nT * 3, nE * 3, nS * 3, nS * 3};
@@ -818,9 +821,9 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   const size_t nS = (size_t)count4(p->scatter_shape);
   auto f4 = [](DeviceBuffer& b) { return (float4*)b.ptr; };
   const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
-  const Dir* hdirs = (const Dir*)ctx->half_dirs.ptr;
   const long long n4 = (long long)nS, e4 = (long long)nE;
-  const int j = kMlpOrders - m;  // seeded partial sum X_j, j >= 1
+  const int N = p->iterations;
+  const int j = N - m;  // seeded partial sum X_j, 1 <= j <= 4
   TRY(enqueue_prologue(ctx, p, P, nullptr, timer, merged));
   // the network's order sums: P_1..P_4 into pred[0..3] (the ones below the seed feed the final
   // table's re-sampling chain), its P_5 into dS[1] (residual)
@@ -883,17 +886,17 @@ int enqueue_seeded(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevPar
   // tables of T, E and M1 (atmosphere_lut.clj:98-105)
   const float4* S_cur = f4(ctx->R1);
   int s_next = 0;
-  for (int k = 1; k <= kMlpOrders; ++k) {
+  for (int k = 1; k <= N; ++k) {
     ResampleJob jobs[kMaxResampleJobs];
     int nj = 0;
     jobs[nj++] = ResampleJob{4, S_cur, f4(ctx->pred[k - 1]), f4(ctx->Sacc[s_next]), nullptr, 0,
                              k > 1 ? f4(ctx->pred[k - 2]) : nullptr};
-    if (k == kMlpOrders) {
+    if (k == N) {
       jobs[nj++] = ResampleJob{2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, nullptr};
       jobs[nj++] = ResampleJob{3, f4(ctx->dE[1]), nullptr, nullptr, (float*)ctx->out[1].ptr, 0, nullptr};
       jobs[nj++] = ResampleJob{4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, nullptr};
     }
-    LAUNCH(timer, k == kMlpOrders ? kFinal : kAccumulate, launch_resamples(P, jobs, nj, ctx->stream));
+    LAUNCH(timer, k == N ? kFinal : kAccumulate, launch_resamples(P, jobs, nj, ctx->stream));
     S_cur = f4(ctx->Sacc[s_next]);
     s_next = 1 - s_next;
   }
@@ -1118,10 +1121,10 @@ int sfsim_atmo_destroy(sfsim_atmo_ctx* ctx) {
                           &ctx->geom,   &ctx->slab,    &ctx->slab_mie};
   for (DeviceBuffer* b : bufs) release(*b);
   DeviceBuffer* more[] = {&ctx->codes,   &ctx->targets, &ctx->sumS,    &ctx->cap[0],  &ctx->cap[1],
-                          &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->pred[0], &ctx->pred[1],
-                          &ctx->pred[2], &ctx->pred[3], &ctx->pred[4], &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sunmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum[0], &ctx->Ssum[1],
+                          &ctx->cap[2],  &ctx->cap[3],  &ctx->cap[4],  &ctx->resid,  &ctx->sun_depth, &ctx->ray_geo, &ctx->rmap, &ctx->sunmap, &ctx->sgeo, &ctx->ray_order, &ctx->Ssum[0], &ctx->Ssum[1],
                           &ctx->batch_mem, &ctx->batch_params};
   for (DeviceBuffer* b : more) release(*b);
+  for (DeviceBuffer& b : ctx->pred) release(b);
   surrogate_free(ctx->sg);
   drop_graph(ctx);
   drop_seeded_graph(ctx);
@@ -1542,7 +1545,8 @@ int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atm
   const sfsim_atmo_params& p0 = atmospheres[0];
   for (int a = 0; a < count; ++a) {
     TRY(validate(ctx, &atmospheres[a]));
-    if (atmospheres[a].iterations != 5) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "training needs 5-order builds");
+    if (atmospheres[a].iterations < kMlpOrders)
+      return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "training needs builds of at least 5 orders");
     if (std::memcmp(atmospheres[a].scatter_shape, p0.scatter_shape, sizeof(p0.scatter_shape)) != 0)
       return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "all training atmospheres need the same scatter shape");
   }
@@ -1556,7 +1560,9 @@ int sfsim_atmo_surrogate_train(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* atm
   float4* caps[kMlpOrders];
   for (int k = 0; k < kMlpOrders; ++k) caps[k] = (float4*)ctx->cap[k].ptr;
   for (int a = 0; a < count; ++a) {
-    TRY(run_pipeline(ctx, &atmospheres[a], nullptr, false, caps));
+    sfsim_atmo_params five = atmospheres[a];  // P_1..P_5 are the first five orders of any longer build
+    five.iterations = kMlpOrders;
+    TRY(run_pipeline(ctx, &five, nullptr, false, caps));
     HIP_TRY(ctx, surrogate_make_targets(caps, (long long)nS, (float*)ctx->targets.ptr + (size_t)a * nS * kMlpUsed,
                                         ctx->stream));
     codes[a] = code_of(&atmospheres[a]);
@@ -1621,7 +1627,7 @@ int sfsim_atmo_build_from_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, 
 int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float* sums) {
   TRY(activate(ctx));
   TRY(validate(ctx, p));
-  if (p->iterations != kMlpOrders) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "order sums need a 5-order build");
+  if (p->iterations < kMlpOrders) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "order sums need a build of at least 5 orders");
   if (!sums) return fail(ctx, SFSIM_ATMO_ERR_ARGUMENT, "sums is NULL");
   const size_t nS = (size_t)count4(p->scatter_shape);
   TRY(ensure(ctx, ctx->sumS, nS * sizeof(float4)));
@@ -1630,7 +1636,9 @@ int sfsim_atmo_order_sums(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, float
     TRY(ensure(ctx, ctx->cap[k], nS * sizeof(float4)));
     caps[k] = (float4*)ctx->cap[k].ptr;
   }
-  TRY(run_pipeline(ctx, p, nullptr, false, caps));
+  sfsim_atmo_params five = *p;
+  five.iterations = kMlpOrders;
+  TRY(run_pipeline(ctx, &five, nullptr, false, caps));
   for (int k = 0; k < kMlpOrders; ++k) TRY(download_table(ctx, caps[k], sums + (size_t)k * nS * 3, nS));
   return 0;
 }

@@ -24,6 +24,7 @@ constexpr int kMlpHidden = 128;
 constexpr int kMlpOut = 16;
 constexpr int kMlpOrders = 5;
 constexpr int kMlpUsed = 3 * kMlpOrders;
+constexpr int kMaxSeededOrders = 32;  // scattering orders of a seeded build (the network predicts the first 5)
 constexpr float kMlpEps = 1e-8f;     // floor inside the log
 constexpr float kMlpShift = 10.0f;   // targets are log(P + eps) + shift
 constexpr int kMaxSplits = 32;       // K splits of the weight-gradient products
