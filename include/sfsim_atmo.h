@@ -176,6 +176,21 @@ int sfsim_atmo_fetch_batch(sfsim_atmo_ctx* ctx, int atmosphere, int which, float
 int sfsim_atmo_stage_timings(const sfsim_atmo_ctx* ctx, double* ms, int* launches, int n);
 const char* sfsim_atmo_stage_name(int kind);
 
+/* Progress of a build (util.clj:441-449 progress-wrap and the "Iteration k/N" line of
+ * atmosphere_lut.clj:80, which the reference prints while it evaluates the tables): fn(user, stage,
+ * iteration, iterations) is called once the device has finished each stage of a build, in order, on a
+ * thread of the HIP runtime (it must not call into the library or HIP): SFSIM_ATMO_PROGRESS_FIRST_ORDER
+ * after the transmittance, surface-radiance-base and single-scattering tables (iteration 0), then
+ * SFSIM_ATMO_PROGRESS_ITERATION after each scattering order (iteration 1..iterations), then
+ * SFSIM_ATMO_PROGRESS_FINAL after the final tables (iteration 0).  Applies to generate_luts, build,
+ * build_timed and build_batch of this context; while a callback is set, build launches the stages
+ * directly instead of replaying its graph (same results).  fn NULL removes the callback. */
+#define SFSIM_ATMO_PROGRESS_FIRST_ORDER 0
+#define SFSIM_ATMO_PROGRESS_ITERATION 1
+#define SFSIM_ATMO_PROGRESS_FINAL 2
+typedef void (*sfsim_atmo_progress_fn)(void* user, int stage, int iteration, int iterations);
+int sfsim_atmo_set_progress(sfsim_atmo_ctx* ctx, sfsim_atmo_progress_fn fn, void* user);
+
 /* Work the kernels perform for one build of p (extinction samples, 4D and 2D lookups), counted on
  * the device by an instrumented build (not the timed one).  counts: [0] extinction samples of the
  * single-scatter kernel, [1] extinction samples of all kernels, [2] 4D lookups, [3] 2D lookups. */

@@ -32,8 +32,12 @@ EXPORTS = (
     "sfsim_atmo_surrogate_train", "sfsim_atmo_build_seeded", "sfsim_atmo_build_from_sums", "sfsim_atmo_order_sums",
     "sfsim_atmo_surrogate_predict", "sfsim_atmo_set_sun_table", "sfsim_atmo_get_sun_table",
     "sfsim_atmo_generate_luts_noalloc", "sfsim_atmo_build_batch", "sfsim_atmo_fetch_batch",
-    "sfsim_atmo_standard_atmosphere", "sfsim_atmo_speed_of_sound",
+    "sfsim_atmo_standard_atmosphere", "sfsim_atmo_speed_of_sound", "sfsim_atmo_set_progress",
 )
+
+# sfsim_atmo_set_progress stages and the callback type
+PROGRESS_FIRST_ORDER, PROGRESS_ITERATION, PROGRESS_FINAL = 0, 1, 2
+ProgressFn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 
 
 class AtmoError(RuntimeError):
@@ -146,6 +150,7 @@ def lib():
         L.sfsim_atmo_build_timed.argtypes = [vp, pp]
         L.sfsim_atmo_fetch.argtypes = [vp, ctypes.c_int, fp, ctypes.c_size_t]
         L.sfsim_atmo_build_batch.argtypes = [vp, pp, ctypes.c_int]
+        L.sfsim_atmo_set_progress.argtypes = [vp, ProgressFn, vp]
         L.sfsim_atmo_fetch_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_size_t]
         L.sfsim_atmo_stage_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                                                ctypes.c_int]
@@ -374,6 +379,12 @@ class Atmosphere:
         out = np.zeros(shapes[which], np.float32)
         self._check(lib().sfsim_atmo_fetch_batch(self._ctx, atmosphere, which, _fp(out), out.size))
         return out
+
+    def set_progress(self, fn):
+        """sfsim_atmo_set_progress: fn(stage, iteration, iterations) after each stage of later builds (called
+        on a thread of the HIP runtime; keep it short); None removes it."""
+        self._progress = None if fn is None else ProgressFn(lambda user, stage, it, n: fn(stage, it, n))
+        self._check(lib().sfsim_atmo_set_progress(self._ctx, self._progress or ProgressFn(), None))
 
     def stage_timings(self):
         """{kernel kind: (milliseconds, launches)} of the last build_timed() or seeded build."""

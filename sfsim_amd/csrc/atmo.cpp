@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -84,10 +85,22 @@ hipError_t scratch_address(int device, void** out) {
 
 }  // namespace
 
+// One progress report queued on the stream (sfsim_atmo_set_progress); the runtime calls the host function
+// with its address once the stream reaches it.
+struct ProgressNote {
+  sfsim_atmo_progress_fn fn;
+  void* user;
+  int stage, iteration, iterations;
+};
+
 struct sfsim_atmo_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string error;
+  // progress callback and the reports of the build in flight (a deque keeps their addresses stable)
+  sfsim_atmo_progress_fn progress_fn = nullptr;
+  void* progress_user = nullptr;
+  std::deque<ProgressNote> notes;
   // device address of the scratch array on this context's device (sfsim_atmo_generate_luts_noalloc)
   void* scratch = nullptr;
   // set on the stack contexts of batched builds and of sfsim_atmo_generate_luts_noalloc: every buffer is
@@ -609,6 +622,19 @@ int enqueue_prologue(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
   return 0;
 }
 
+void progress_host(void* arg) {
+  const ProgressNote* n = (const ProgressNote*)arg;
+  n->fn(n->user, n->stage, n->iteration, n->iterations);
+}
+
+// Queues a progress report behind the launches enqueued so far (nothing without a callback).
+int note_progress(sfsim_atmo_ctx* ctx, int stage, int iteration, int iterations) {
+  if (!ctx->progress_fn) return 0;
+  ctx->notes.push_back(ProgressNote{ctx->progress_fn, ctx->progress_user, stage, iteration, iterations});
+  HIP_TRY(ctx, hipLaunchHostFunc(ctx->stream, progress_host, &ctx->notes.back()));
+  return 0;
+}
+
 int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevParams& P, WorkCounters* wc,
                      Timer& timer, float4* const* capture, bool merged = false) {
   const size_t nE = (size_t)count2(p->surface_radiance_shape);
@@ -618,6 +644,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
   const Dir* sdirs = (const Dir*)ctx->sphere_dirs.ptr;
 
   TRY(enqueue_prologue(ctx, p, P, wc, timer, merged));
+  TRY(note_progress(ctx, SFSIM_ATMO_PROGRESS_FIRST_ORDER, 0, p->iterations));
   // dS <- R1 + M1 * phase (a function); S <- R1; E <- 0; dE <- E0
   // The accumulations E <- table(E + dE), S <- table(S + dS) (atmosphere_lut.clj:95-97) ride in the
   // ray-scatter launches: iteration k writes Ssum[k % 2] = S + dS_k, with S = R1 for the first and
@@ -668,6 +695,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
       if (order <= kMlpOrders)
         HIP_TRY(ctx, hipMemcpyAsync(capture[order - 1], sum, nS * sizeof(float4), hipMemcpyDeviceToDevice, ctx->stream));
     }
+    TRY(note_progress(ctx, SFSIM_ATMO_PROGRESS_ITERATION, it + 1, p->iterations));
   }
   // the last S re-sampling and the final make-lookup-table calls and file layouts
   // (atmosphere_lut.clj:97-105), the independent ones in one launch: T, E and M1 beside S <- table(S + dS)
@@ -689,6 +717,7 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
     for (int a = 0; a < std::max(P.batch, 1); ++a)
       LAUNCH(timer, kFinal, launch_fill_zero((float*)((char*)ctx->out[1].ptr + (size_t)a * P.atm_stride),
                                              (long long)ctx->out_count[1], ctx->stream));
+  TRY(note_progress(ctx, SFSIM_ATMO_PROGRESS_FINAL, 0, p->iterations));
   return 0;
 }
 
@@ -698,6 +727,10 @@ int run_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, WorkCounters* 
   TRY(prepare_pipeline(ctx, p));
   const DevParams P = dev_params(ctx, p);
   Timer timer{ctx, timed};
+  if (!ctx->notes.empty()) {  // reports of an earlier build, run by the time its stream is idle
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->notes.clear();
+  }
   TRY(enqueue_pipeline(ctx, p, P, wc, timer, capture));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   TRY(timer.collect());
@@ -1285,7 +1318,15 @@ int sfsim_atmo_resample_table(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, i
 
 int sfsim_atmo_build(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
   TRY(activate(ctx));
+  if (ctx->progress_fn) return run_pipeline(ctx, p, nullptr, false);  // host reports between the stages
   return run_pipeline_graph(ctx, p);
+}
+
+int sfsim_atmo_set_progress(sfsim_atmo_ctx* ctx, sfsim_atmo_progress_fn fn, void* user) {
+  if (!ctx) return SFSIM_ATMO_ERR_ARGUMENT;
+  ctx->progress_fn = fn;
+  ctx->progress_user = fn ? user : nullptr;
+  return 0;
 }
 
 int sfsim_atmo_build_timed(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p) {
@@ -1386,6 +1427,8 @@ int sfsim_atmo_build_batch(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* ps, int
   sh.no_alloc = true;
   sh.sun_rows = ctx->sun_rows;
   sh.sun_cols = ctx->sun_cols;
+  sh.progress_fn = ctx->progress_fn;
+  sh.progress_user = ctx->progress_user;
   const size_t shared = layout_shared(&sh, p0, nullptr), block = layout_atmosphere(&sh, p0, nullptr, true);
   TRY(ensure(ctx, ctx->batch_mem, shared + (size_t)count * block));
   TRY(ensure(ctx, ctx->batch_params, (size_t)count * sizeof(DevParams)));
@@ -1414,8 +1457,10 @@ int sfsim_atmo_build_batch(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* ps, int
                               hipMemcpyHostToDevice, ctx->stream));
   Timer timer{&sh, false};
   rc = enqueue_pipeline(&sh, p0, host[0], nullptr, timer, nullptr, true);
+  // the progress reports queued so far point into sh: wait for them on the failure path as well
+  const hipError_t es = hipStreamSynchronize(ctx->stream);
   if (rc) return fail_sh(rc);
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, es);
   ctx->batch_count = count;
   ctx->batch_stride = block;
   for (int i = 0; i < 4; ++i) {
