@@ -139,7 +139,7 @@ int sfsim_atmo_generate_luts(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, fl
  * the library runs on holds until the process ends.  The call allocates and frees no device memory; it
  * copies the quadrature directions and the ray order from the host and returns after the stream has drained.  A
  * workload whose intermediate tables do not fit the array is rejected with SFSIM_ATMO_ERR_ARGUMENT (the
- * reference's production workload takes 265 MiB of it, the 32x64x32x8 5-order workload 138 MiB).  All
+ * reference's production workload takes 277 MiB of it, the 32x64x32x8 5-order workload 142 MiB).  All
  * calls on one GPU share its array, which keeps the last call's contents: calls that overlap on one GPU
  * (several contexts or threads) race and must be serialised by the caller. */
 #define SFSIM_ATMO_NOALLOC_SCRATCH_BYTES ((size_t)288 << 20)

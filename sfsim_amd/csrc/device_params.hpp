@@ -119,12 +119,21 @@ __host__ __device__ inline size_t resample_map_bytes(const DevParams& P) {
 
 // Views of DevParams::ray_geo: m = NH x NE x ray_steps samples, sample i of view ray (ih, ie) at
 // (ih NE + ie) ray_steps + i; t_i = (i + 1/2) D / ray_steps along the ray from the grid point.
+// The ray-scatter fold of sample p_i: the element offsets in the dJ table of the (height, elevation) nodes
+// of its bilinear cell by fold slot (slot 2 (h & 1) + (e & 1); -1 where the slot is unused at the upper
+// edge of the grid) and their weights (zero for unused slots).
+struct FoldStep {
+  int4 off;
+  float4 wt;
+};
+
 struct RayGeoView {
-  double2* th;  // t_i, altitude h_i
-  float4* T;    // T(x, p_i) rgb, w: exp(-h_i / H_mie)
-  float4* rs;   // height coordinate, elevation coordinate of v, alpha_i, beta_i (sun-elevation coordinate terms)
-  float4* ss;   // Rt^2 - r_i^2, D_min, (sun_cols - 1) / (D_max - D_min), exp(-h_i / H_rayleigh) (sun table)
-  float2* row;  // sun-table row fraction, bits of the int index of the row start
+  double2* th;    // t_i, altitude h_i
+  float4* T;      // T(x, p_i) rgb, w: exp(-h_i / H_mie)
+  float4* rs;     // height coordinate, elevation coordinate of v, alpha_i, beta_i (sun-elevation coordinate terms)
+  float4* ss;     // Rt^2 - r_i^2, D_min, (sun_cols - 1) / (D_max - D_min), exp(-h_i / H_rayleigh) (sun table)
+  FoldStep* fold; // the dJ nodes and weights of the ray-scatter fold
+  float2* row;    // sun-table row fraction, bits of the int index of the row start
 };
 
 __host__ __device__ inline size_t ray_geo_samples(const DevParams& P) {
@@ -132,7 +141,7 @@ __host__ __device__ inline size_t ray_geo_samples(const DevParams& P) {
 }
 
 __host__ __device__ inline size_t ray_geometry_bytes(const DevParams& P) {
-  return ray_geo_samples(P) * (sizeof(double2) + 3 * sizeof(float4) + sizeof(float2));
+  return ray_geo_samples(P) * (sizeof(double2) + 3 * sizeof(float4) + sizeof(FoldStep) + sizeof(float2));
 }
 
 __host__ __device__ inline RayGeoView ray_geo_view(const DevParams& P) {
@@ -147,6 +156,8 @@ __host__ __device__ inline RayGeoView ray_geo_view(const DevParams& P) {
   b += m * sizeof(float4);
   v.ss = (float4*)b;
   b += m * sizeof(float4);
+  v.fold = (FoldStep*)b;
+  b += m * sizeof(FoldStep);
   v.row = (float2*)b;
   return v;
 }

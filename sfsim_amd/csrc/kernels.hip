@@ -600,9 +600,23 @@ __device__ __forceinline__ void ray_geometry_body(const DevParams& P, WorkCounte
   const double k3 = -3.0 * 1.4426950408889634;
   V.th[g] = make_double2(t, h);
   V.T[g] = make_float4(T.x, T.y, T.z, (float)exp(-h * P.inv_scale_mie));
-  V.rs[g] = make_float4((float)altitude_to_index(P.g, NH, h),
-                        (float)elevation_to_index(P.g, NE, r, ray.pd0 + t, ray.above), (float)(k3 / r),
-                        (float)(k3 * t / r));
+  const float hc = (float)altitude_to_index(P.g, NH, h);
+  const float ec = (float)elevation_to_index(P.g, NE, r, ray.pd0 + t, ray.above);
+  V.rs[g] = make_float4(hc, ec, (float)(k3 / r), (float)(k3 * t / r));
+  {  // the fold's dJ nodes by slot: the corner with height parity ph and elevation parity pe goes to slot
+     // 2 ph + pe (at the upper edge of the grid, where i1 == i0 and the weight of i1 is zero, the
+     // coinciding corners share one slot and the slots of the other parity stay unused)
+    const int NLA = P.nS[2] * P.nS[3];
+    const Lerp Lh = make_lerp(hc, NH), Le = make_lerp(ec, NE);
+    int hs[2], es[2];
+    float hw[2], ew[2];
+    corner_by_parity(Lh, hs, hw);
+    corner_by_parity(Le, es, ew);
+    auto off = [&](int ph, int pe) { return hs[ph] < 0 || es[pe] < 0 ? -1 : (hs[ph] * NE + es[pe]) * NLA; };
+    auto wt = [&](int ph, int pe) { return hs[ph] < 0 || es[pe] < 0 ? 0.0f : ew[pe] * hw[ph]; };
+    V.fold[g] = FoldStep{make_int4(off(0, 0), off(0, 1), off(1, 0), off(1, 1)),
+                         make_float4(wt(0, 0), wt(0, 1), wt(1, 0), wt(1, 1))};
+  }
   if (P.sun_rows > 0) {
     const int cols = P.sun_cols;
     const double rho = sqrt(h * (2.0 * P.g.R + h));
@@ -773,7 +787,7 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
 // thread's entry.  The four corners of a cell have distinct parities, so every sample's corners occupy
 // the four slots, and a corner shared with the previous sample is already in its slot: a slot is
 // re-read only when its node changes (about 60 node reads per ray of 64 samples instead of four per
-// sample that changes cell).  Which slots a sample re-reads is worked out once per launch (s_mask).
+// sample that changes cell).  The nodes and weights of every sample come from the per-build ray geometry.
 struct FoldSlots {
   float4 v[4];
 };
@@ -783,49 +797,54 @@ __device__ __forceinline__ void fold_slots_init(FoldSlots& f) {
 }
 
 // Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1):
-// slab[q][kk] = sum over slots of weight x dJ[node][kk].  s_off holds per sample the element offset of
-// each slot's node (-1 when the slot is unused), s_wt their weights (zero for unused slots), s_mask the
-// slots whose node differs from the previous sample's (bit s); all three are workgroup-uniform, so are
-// the branches.  With one entry per thread the slots carry over from chunk to chunk.
-__device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const int4* s_off, const float4* s_wt,
-                                                 const int* s_mask, float4* slab, int i0, int qn, int NLA, int t,
-                                                 int nt, FoldSlots& f) {
+// slab[q][kk] = sum over slots of weight x dJ[node][kk].  fs: the ray's per-build fold steps (node offsets
+// and weights by slot; workgroup-uniform, read with scalar loads, so are the branches); prev: the previous
+// sample's node offsets.  With one entry per thread the slots carry over from sample to sample and a slot
+// is re-read only where its node changed.
+__device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const FoldStep* __restrict__ fs,
+                                                 float4* slab, int i0, int qn, int NLA, int t, int nt, FoldSlots& f,
+                                                 int4& prev) {
   if (NLA <= nt) {
     const int kk = t;
     // lanes past the table read the last entry and store the same bits as the last entry's lane to the
     // same place: no branch around the store, which would give the loop a path that skips the wait for
     // the slot loads and make the next sample's loads wait for them one by one
     const int kl = min(kk, NLA - 1);
+    FoldStep st = fs[i0];
     for (int q = 0; q < qn; ++q) {
-      const int m = __builtin_amdgcn_readfirstlane(s_mask[i0 + q]);
-      const float4 w = s_wt[i0 + q];
-      if (m) {
-        const int4 o = s_off[i0 + q];
-        if (m & 1) f.v[0] = dJ[__builtin_amdgcn_readfirstlane(o.x) + kl];
-        if (m & 2) f.v[1] = dJ[__builtin_amdgcn_readfirstlane(o.y) + kl];
-        if (m & 4) f.v[2] = dJ[__builtin_amdgcn_readfirstlane(o.z) + kl];
-        if (m & 8) f.v[3] = dJ[__builtin_amdgcn_readfirstlane(o.w) + kl];
-      }
+      // the next sample's step is requested before this sample's loads (its latency hides behind them)
+      const FoldStep nx = fs[i0 + min(q + 1, qn - 1)];
+      const int4 o = st.off;
+      if (o.x >= 0 && o.x != prev.x) f.v[0] = dJ[o.x + kl];
+      if (o.y >= 0 && o.y != prev.y) f.v[1] = dJ[o.y + kl];
+      if (o.z >= 0 && o.z != prev.z) f.v[2] = dJ[o.z + kl];
+      if (o.w >= 0 && o.w != prev.w) f.v[3] = dJ[o.w + kl];
+      prev = o;
+      const float4 w = st.wt;
       const float4 a = f.v[0], b = f.v[1], c = f.v[2], d = f.v[3];
       slab[q * NLA + kl] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
                                        fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
                                        fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+      st = nx;
     }
   } else {
+    FoldStep st = fs[i0];
     for (int q = 0; q < qn; ++q) {
-      const int4 o = s_off[i0 + q];
-      const float4 w = s_wt[i0 + q];
+      const FoldStep nx = fs[i0 + min(q + 1, qn - 1)];
+      const int4 o = st.off;
+      const float4 w = st.wt;
       // an unused slot (offset -1, weight 0) reads node 0, which adds zero
-      const float4* __restrict__ d0 = dJ + max(__builtin_amdgcn_readfirstlane(o.x), 0);
-      const float4* __restrict__ d1 = dJ + max(__builtin_amdgcn_readfirstlane(o.y), 0);
-      const float4* __restrict__ d2 = dJ + max(__builtin_amdgcn_readfirstlane(o.z), 0);
-      const float4* __restrict__ d3 = dJ + max(__builtin_amdgcn_readfirstlane(o.w), 0);
+      const float4* __restrict__ d0 = dJ + max(o.x, 0);
+      const float4* __restrict__ d1 = dJ + max(o.y, 0);
+      const float4* __restrict__ d2 = dJ + max(o.z, 0);
+      const float4* __restrict__ d3 = dJ + max(o.w, 0);
       for (int kk = t; kk < NLA; kk += nt) {
         const float4 a = d0[kk], b = d1[kk], c = d2[kk], d = d3[kk];
         slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
                                          fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
                                          fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
       }
+      st = nx;
     }
   }
 }
@@ -848,26 +867,22 @@ constexpr int kRSHalf = kRSBlock / 2;
 template <int EPT, bool kB>
 __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) ray_scatter_kernel(
     const DevParams P0, const DevParams* __restrict__ PB, const float4* __restrict__ dJ,
-    const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs, float4* __restrict__ out,
-    const RsAccum accum_arg, int Q, WorkCounters* wc) {
+    const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs, const FoldStep* __restrict__ geo_fold,
+    float4* __restrict__ out, const RsAccum accum_arg, int Q, WorkCounters* wc) {
   const DevParams& P = params_of<kB>(P0, PB);
   if constexpr (kB) {  // this atmosphere's tables and ray geometry (the accumulation tables where they are used);
                        // offsets of the restrict arguments, whose loads stay provably unclobbered and uniform
     geo_T = atm_ptr<kB>(geo_T, P0);
     geo_rs = atm_ptr<kB>(geo_rs, P0);
+    geo_fold = atm_ptr<kB>(geo_fold, P0);
     dJ = atm_ptr<kB>(dJ, P);
     out = atm_ptr<kB>(out, P);
   }
-  // dynamic LDS: 2 x Q x (NL x NA) slab entries (chunk c in half c % 2), then per sample the slab
-  // offsets (in units of NL x NA), their bilinear weights and the fold slots whose node differs from the
-  // previous sample's
+  // dynamic LDS: 2 x Q x (NL x NA) slab entries (chunk c in half c % 2)
   extern __shared__ float4 s_slab[];
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   const int NLA = NL * NA;
   const int n = P.ray_steps;
-  int4* s_off = (int4*)(s_slab + 2 * Q * NLA);
-  float4* s_wt = (float4*)(s_off + n);
-  int* s_mask = (int*)(s_wt + n);
   if ((int)blockIdx.x >= NH * NE) {  // the extra workgroup: E <- tab(dE + E) (atmosphere_lut.clj:95-96)
     const long long nE = (long long)P.nE[0] * P.nE[1];
     const float4* dE = atm_ptr<kB>(accum_arg.dE, P);
@@ -891,41 +906,33 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
   // coordinate, alpha_i, beta_i)
   const float4* __restrict__ gT = geo_T + (long long)rayid * n;
   const float4* __restrict__ gRS = geo_rs + (long long)rayid * n;
-  // element offsets of the (height, elevation) slabs of every sample and their bilinear weights, by
-  // fold slot: the corner with height index parity ph and elevation index parity pe goes to slot
-  // 2 ph + pe (at the upper edge of the grid, where i1 == i0 and the weight of i1 is zero, the
-  // coinciding corners share one slot and the slots of the other parity stay unused)
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const float4 g = gRS[i];
-    const Lerp Lh = make_lerp(g.x, NH), Le = make_lerp(g.y, NE);
-    int hs[2], es[2];
-    float hw[2], ew[2];
-    corner_by_parity(Lh, hs, hw);
-    corner_by_parity(Le, es, ew);
-    auto off = [&](int ph, int pe) { return hs[ph] < 0 || es[pe] < 0 ? -1 : (hs[ph] * NE + es[pe]) * NLA; };
-    auto wt = [&](int ph, int pe) { return hs[ph] < 0 || es[pe] < 0 ? 0.0f : ew[pe] * hw[ph]; };
-    s_off[i] = make_int4(off(0, 0), off(0, 1), off(1, 0), off(1, 1));
-    s_wt[i] = make_float4(wt(0, 0), wt(0, 1), wt(1, 0), wt(1, 1));
-  }
-  __syncthreads();
-  // a slot is read when it holds a node that the previous sample's same slot did not (an unused slot
-  // keeps its old value, weighted zero)
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int4 o = s_off[i], p = i > 0 ? s_off[i - 1] : make_int4(-1, -1, -1, -1);
-    s_mask[i] = (o.x >= 0 && o.x != p.x ? 1 : 0) | (o.y >= 0 && o.y != p.y ? 2 : 0) | (o.z >= 0 && o.z != p.z ? 4 : 0) |
-                (o.w >= 0 && o.w != p.w ? 8 : 0);
-  }
-  __syncthreads();
   const int chunks = (n + Q - 1) / Q;
+  // with one entry per lane the fold waves, idle during the last chunk, re-sample S for the epilogue
+  // (0.5% per launch); with several entries per lane it is slower there and stays with the lookup waves
+  constexpr bool kFoldResamples = EPT == 1;
   if (folder) {
+    const FoldStep* __restrict__ gF = geo_fold + (long long)rayid * n;
     FoldSlots fs;
     fold_slots_init(fs);
-    ray_scatter_fold(dJ, s_off, s_wt, s_mask, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs);
+    int4 prev = make_int4(-1, -1, -1, -1);
+    ray_scatter_fold(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, prev);
     __syncthreads();
     for (int c = 0; c < chunks; ++c) {
-      if (c + 1 < chunks)
-        ray_scatter_fold(dJ, s_off, s_wt, s_mask, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q,
-                         min(Q, n - (c + 1) * Q), NLA, t, kRSHalf, fs);
+      if (c + 1 < chunks) {
+        ray_scatter_fold(dJ, gF, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA, t,
+                         kRSHalf, fs, prev);
+      } else if (kFoldResamples && accum_arg.S_out) {
+        // while the last chunk's lookups run: the running sum S of atmosphere_lut.clj:95-97 at this ray's
+        // grid points into the half that chunk c - 1's lookups have finished with, for the lookup waves to
+        // add their dS to after the barrier
+        const float4* S_in = atm_ptr<kB>(accum_arg.S_in, P);
+        float4* su = s_slab + ((c + 1) & 1) * Q * NLA;
+        const long long base = (long long)rayid * NLA;
+        for (int kk = t; kk < NLA; kk += kRSHalf) {
+          const float3 u = accum_arg.S_resample ? resample4_value_lean(P, S_in, base + kk) : ld3(S_in, base + kk);
+          su[kk] = make_float4(u.x, u.y, u.z, 0.f);
+        }
+      }
       __syncthreads();
     }
     return;
@@ -982,9 +989,15 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     if (k < NLA) {
       const float4 d = make_float4(acc[e].x * a, acc[e].y * a, acc[e].z * a, 0.f);
       out[base + k] = d;
-      if (accum_arg.S_out) {  // the running sum S + dS of atmosphere_lut.clj:95-97 at this grid point
-        const float4* S_in = atm_ptr<kB>(accum_arg.S_in, P);
-        const float3 u = accum_arg.S_resample ? resample4_value_lean(P, S_in, base + k) : ld3(S_in, base + k);
+      if (accum_arg.S_out) {  // S + dS at this grid point
+        float3 u;
+        if (kFoldResamples) {
+          const float4 v = s_slab[(chunks & 1) * Q * NLA + k];
+          u = make_float3(v.x, v.y, v.z);
+        } else {
+          const float4* S_in = atm_ptr<kB>(accum_arg.S_in, P);
+          u = accum_arg.S_resample ? resample4_value_lean(P, S_in, base + k) : ld3(S_in, base + k);
+        }
         atm_ptr<kB>(accum_arg.S_out, P)[base + k] = make_float4(u.x + d.x, u.y + d.y, u.z + d.z, 0.f);
       }
       lookups += n;
@@ -1900,32 +1913,31 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   const unsigned rays = (unsigned)(P.nS[0] * P.nS[1]) + (acc.E_out ? 1u : 0u);
   int nla = P.nS[2] * P.nS[3];
   int Q = ray_scatter_chunk(P);
-  const size_t shmem =
-      2 * sizeof(float4) * (size_t)Q * nla + (size_t)P.ray_steps * (sizeof(int4) + sizeof(float4) + sizeof(int));
+  const size_t shmem = 2 * sizeof(float4) * (size_t)Q * nla;
   const RayGeoView V = ray_geo_view(P);
   const DevParams* PB = P.batch_params;
   if (nla > 4 * kRSHalf) return hipErrorInvalidValue;
   if (P.batch > 0) {
     const dim3 grid(rays, 1, (unsigned)P.batch);
     if (nla <= kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<1, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs, out,
-                         acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<1, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
+                         V.fold, out, acc, Q, wc);
     else if (nla <= 2 * kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<2, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs, out,
-                         acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<2, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
+                         V.fold, out, acc, Q, wc);
     else
-      hipLaunchKernelGGL((ray_scatter_kernel<4, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs, out,
-                         acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<4, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
+                         V.fold, out, acc, Q, wc);
   } else {
     if (nla <= kRSHalf)
       hipLaunchKernelGGL((ray_scatter_kernel<1, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         out, acc, Q, wc);
+                         V.fold, out, acc, Q, wc);
     else if (nla <= 2 * kRSHalf)
       hipLaunchKernelGGL((ray_scatter_kernel<2, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         out, acc, Q, wc);
+                         V.fold, out, acc, Q, wc);
     else
       hipLaunchKernelGGL((ray_scatter_kernel<4, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         out, acc, Q, wc);
+                         V.fold, out, acc, Q, wc);
   }
   return hipGetLastError();
 }
