@@ -796,15 +796,24 @@ __device__ __forceinline__ void fold_slots_init(FoldSlots& f) {
   for (int s = 0; s < 4; ++s) f.v[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// The folded value of one entry: the weighted sum of its four slots.
+__device__ __forceinline__ float4 fold_value(const FoldSlots& f, const float4 w) {
+  const float4 a = f.v[0], b = f.v[1], c = f.v[2], d = f.v[3];
+  return make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
+                     fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
+                     fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+}
+
 // Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1):
 // slab[q][kk] = sum over slots of weight x dJ[node][kk].  fs: the ray's per-build fold steps (node offsets
 // and weights by slot; workgroup-uniform, read with scalar loads, so are the branches); prev: the previous
 // sample's node offsets.  With one entry per thread the slots carry over from sample to sample and a slot
 // is re-read only where its node changed.
+template <int KPT>
 __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const FoldStep* __restrict__ fs,
                                                  float4* slab, int i0, int qn, int NLA, int t, int nt, FoldSlots& f,
                                                  int4& prev) {
-  if (NLA <= nt) {
+  if (KPT == 1) {
     const int kk = t;
     // lanes past the table read the last entry and store the same bits as the last entry's lane to the
     // same place: no branch around the store, which would give the loop a path that skips the wait for
@@ -820,17 +829,12 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
       if (o.z >= 0 && o.z != prev.z) f.v[2] = dJ[o.z + kl];
       if (o.w >= 0 && o.w != prev.w) f.v[3] = dJ[o.w + kl];
       prev = o;
-      const float4 w = st.wt;
-      const float4 a = f.v[0], b = f.v[1], c = f.v[2], d = f.v[3];
-      slab[q * NLA + kl] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
-                                       fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
-                                       fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+      slab[q * NLA + kl] = fold_value(f, st.wt);
       st = nx;
     }
-  } else {
-    FoldStep st = fs[i0];
+  } else {  // KPT (even) entries per thread, NLA <= KPT nt: every sample reads its four nodes
     for (int q = 0; q < qn; ++q) {
-      const FoldStep nx = fs[i0 + min(q + 1, qn - 1)];
+      const FoldStep st = fs[i0 + q];
       const int4 o = st.off;
       const float4 w = st.wt;
       // an unused slot (offset -1, weight 0) reads node 0, which adds zero
@@ -838,13 +842,25 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
       const float4* __restrict__ d1 = dJ + max(o.y, 0);
       const float4* __restrict__ d2 = dJ + max(o.z, 0);
       const float4* __restrict__ d3 = dJ + max(o.w, 0);
-      for (int kk = t; kk < NLA; kk += nt) {
-        const float4 a = d0[kk], b = d1[kk], c = d2[kk], d = d3[kk];
-        slab[q * NLA + kk] = make_float4(fmaf(w.w, d.x, fmaf(w.z, c.x, fmaf(w.y, b.x, w.x * a.x))),
-                                         fmaf(w.w, d.y, fmaf(w.z, c.y, fmaf(w.y, b.y, w.x * a.y))),
-                                         fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
+      // two entries per round, eight loads in flight (entries past the table repeat the last one, which
+      // stores the same bits to the same place)
+#pragma unroll
+      for (int e = 0; e < KPT; e += 2) {
+        const int k0 = min(t + e * nt, NLA - 1), k1 = min(t + (e + 1) * nt, NLA - 1);
+        const float4 a0 = d0[k0], b0 = d1[k0], c0 = d2[k0], e0 = d3[k0];
+        const float4 a1 = d0[k1], b1 = d1[k1], c1 = d2[k1], e1 = d3[k1];
+        FoldSlots f0, f1;
+        f0.v[0] = a0;
+        f0.v[1] = b0;
+        f0.v[2] = c0;
+        f0.v[3] = e0;
+        f1.v[0] = a1;
+        f1.v[1] = b1;
+        f1.v[2] = c1;
+        f1.v[3] = e1;
+        slab[q * NLA + k0] = fold_value(f0, w);
+        slab[q * NLA + k1] = fold_value(f1, w);
       }
-      st = nx;
     }
   }
 }
@@ -915,12 +931,12 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     FoldSlots fs;
     fold_slots_init(fs);
     int4 prev = make_int4(-1, -1, -1, -1);
-    ray_scatter_fold(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, prev);
+    ray_scatter_fold<EPT>(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, prev);
     __syncthreads();
     for (int c = 0; c < chunks; ++c) {
       if (c + 1 < chunks) {
-        ray_scatter_fold(dJ, gF, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA, t,
-                         kRSHalf, fs, prev);
+        ray_scatter_fold<EPT>(dJ, gF, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
+                              t, kRSHalf, fs, prev);
       } else if (kFoldResamples && accum_arg.S_out) {
         // while the last chunk's lookups run: the running sum S of atmosphere_lut.clj:95-97 at this ray's
         // grid points into the half that chunk c - 1's lookups have finished with, for the lookup waves to
