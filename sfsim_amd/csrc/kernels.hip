@@ -933,24 +933,24 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     int4 prev = make_int4(-1, -1, -1, -1);
     ray_scatter_fold<EPT>(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, prev);
     __syncthreads();
-    for (int c = 0; c < chunks; ++c) {
-      if (c + 1 < chunks) {
-        ray_scatter_fold<EPT>(dJ, gF, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
+    for (int c = 0; c + 1 < chunks; ++c) {
+      ray_scatter_fold<EPT>(dJ, gF, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
                               t, kRSHalf, fs, prev);
-      } else if (kFoldResamples && accum_arg.S_out) {
-        // while the last chunk's lookups run: the running sum S of atmosphere_lut.clj:95-97 at this ray's
-        // grid points into the half that chunk c - 1's lookups have finished with, for the lookup waves to
-        // add their dS to after the barrier
-        const float4* S_in = atm_ptr<kB>(accum_arg.S_in, P);
-        float4* su = s_slab + ((c + 1) & 1) * Q * NLA;
-        const long long base = (long long)rayid * NLA;
-        for (int kk = t; kk < NLA; kk += kRSHalf) {
-          const float3 u = accum_arg.S_resample ? resample4_value_lean(P, S_in, base + kk) : ld3(S_in, base + kk);
-          su[kk] = make_float4(u.x, u.y, u.z, 0.f);
-        }
-      }
       __syncthreads();
     }
+    if (kFoldResamples && accum_arg.S_out) {
+      // while the last chunk's lookups run: the running sum S of atmosphere_lut.clj:95-97 at this ray's grid
+      // points into the half that the lookups of the chunk before the last have finished with, for the
+      // lookup waves to add their dS to after the barrier (the fold slots are dead by now)
+      const float4* S_in = atm_ptr<kB>(accum_arg.S_in, P);
+      float4* su = s_slab + (chunks & 1) * Q * NLA;
+      const long long base = (long long)rayid * NLA;
+      for (int kk = t; kk < NLA; kk += kRSHalf) {
+        const float3 u = accum_arg.S_resample ? resample4_value_lean(P, S_in, base + kk) : ld3(S_in, base + kk);
+        su[kk] = make_float4(u.x, u.y, u.z, 0.f);
+      }
+    }
+    __syncthreads();
     return;
   }
   const Ray ray = make_ray(P, ih, ie);
