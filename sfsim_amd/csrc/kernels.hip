@@ -796,6 +796,21 @@ __device__ __forceinline__ void fold_slots_init(FoldSlots& f) {
   for (int s = 0; s < 4; ++s) f.v[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// The per-build fold steps of one ray, read-only while a ray-scatter launch runs.  The single-atmosphere
+// kernel reads them through its restrict argument; the batched one through the constant address space,
+// where the compiler cannot otherwise prove them unclobbered and would use vector loads.
+typedef __attribute__((address_space(4))) const int* ConstFoldSteps;
+
+__device__ __forceinline__ FoldStep fold_step(const FoldStep* __restrict__ fs, int i) { return fs[i]; }
+
+__device__ __forceinline__ FoldStep fold_step(ConstFoldSteps fs, int i) {
+  const ConstFoldSteps p = fs + 8 * i;  // FoldStep: 8 dwords
+  FoldStep st;
+  st.off = make_int4(p[0], p[1], p[2], p[3]);
+  st.wt = make_float4(__int_as_float(p[4]), __int_as_float(p[5]), __int_as_float(p[6]), __int_as_float(p[7]));
+  return st;
+}
+
 // The folded value of one entry: the weighted sum of its four slots.
 __device__ __forceinline__ float4 fold_value(const FoldSlots& f, const float4 w) {
   const float4 a = f.v[0], b = f.v[1], c = f.v[2], d = f.v[3];
@@ -809,8 +824,8 @@ __device__ __forceinline__ float4 fold_value(const FoldSlots& f, const float4 w)
 // and weights by slot; workgroup-uniform, read with scalar loads, so are the branches); prev: the previous
 // sample's node offsets.  With one entry per thread the slots carry over from sample to sample and a slot
 // is re-read only where its node changed.
-template <int KPT>
-__device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const FoldStep* __restrict__ fs,
+template <int KPT, class Steps>
+__device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const Steps fs,
                                                  float4* slab, int i0, int qn, int NLA, int t, int nt, FoldSlots& f,
                                                  int4& prev) {
   if (KPT == 1) {
@@ -819,10 +834,10 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
     // same place: no branch around the store, which would give the loop a path that skips the wait for
     // the slot loads and make the next sample's loads wait for them one by one
     const int kl = min(kk, NLA - 1);
-    FoldStep st = fs[i0];
+    FoldStep st = fold_step(fs, i0);
     for (int q = 0; q < qn; ++q) {
       // the next sample's step is requested before this sample's loads (its latency hides behind them)
-      const FoldStep nx = fs[i0 + min(q + 1, qn - 1)];
+      const FoldStep nx = fold_step(fs, i0 + min(q + 1, qn - 1));
       const int4 o = st.off;
       if (o.x >= 0 && o.x != prev.x) f.v[0] = dJ[o.x + kl];
       if (o.y >= 0 && o.y != prev.y) f.v[1] = dJ[o.y + kl];
@@ -834,7 +849,7 @@ __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, 
     }
   } else {  // KPT (even) entries per thread, NLA <= KPT nt: every sample reads its four nodes
     for (int q = 0; q < qn; ++q) {
-      const FoldStep st = fs[i0 + q];
+      const FoldStep st = fold_step(fs, i0 + q);
       const int4 o = st.off;
       const float4 w = st.wt;
       // an unused slot (offset -1, weight 0) reads node 0, which adds zero
@@ -927,7 +942,11 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
   // (0.5% per launch); with several entries per lane it is slower there and stays with the lookup waves
   constexpr bool kFoldResamples = EPT == 1;
   if (folder) {
-    const FoldStep* __restrict__ gF = geo_fold + (long long)rayid * n;
+    const FoldStep* __restrict__ gS = geo_fold + (long long)rayid * n;
+    const auto gF = [&] {
+      if constexpr (kB) return (ConstFoldSteps)gS;
+      else return gS;
+    }();
     FoldSlots fs;
     fold_slots_init(fs);
     int4 prev = make_int4(-1, -1, -1, -1);
