@@ -905,7 +905,8 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
                        // offsets of the restrict arguments, whose loads stay provably unclobbered and uniform
     geo_T = atm_ptr<kB>(geo_T, P0);
     geo_rs = atm_ptr<kB>(geo_rs, P0);
-    geo_fold = atm_ptr<kB>(geo_fold, P0);
+    // the fold steps depend only on the planet and the grid, which the atmospheres of a batch share: all
+    // read atmosphere 0's copy (one copy in the caches instead of one per atmosphere)
     dJ = atm_ptr<kB>(dJ, P);
     out = atm_ptr<kB>(out, P);
   }
