@@ -100,7 +100,9 @@ def make_params(planet=None, mie=None, rayleigh=None, transmittance_shape=(64, 2
 def workload(name):
     """Named workloads.  ``production``: the reference's own parameters (atmosphere_lut.clj:20-62).
     ``earth100-64``: the 100 km / 64-step transmittance grid; ``full-5-order``: the 5-order set at
-    T 64x256, E 16x64, S 32x64x32x8, 64 ray steps, 14 sphere steps (66 directions), g = 0.76."""
+    T 64x256, E 16x64, S 32x64x32x8, 64 ray steps, 14 sphere steps (66 directions), g = 0.76;
+    ``large-10-order``: S 64x128x64x16, T 128x512, E 16x64 (no size is given for it beside the entry count of
+    16,843,776 per set, which E 16x64 makes), 128 ray steps, 28 sphere steps (256 directions), 10 orders."""
     earth100 = {"height": 100000.0}
     if name == "production":
         return make_params()
@@ -111,7 +113,7 @@ def workload(name):
         return make_params(planet=earth100, transmittance_shape=(64, 256), surface_radiance_shape=(16, 64),
                            scatter_shape=(32, 64, 32, 8), ray_steps=64, sphere_steps=14, iterations=5)
     if name == "large-10-order":
-        return make_params(planet=earth100, transmittance_shape=(128, 512), surface_radiance_shape=(32, 128),
+        return make_params(planet=earth100, transmittance_shape=(128, 512), surface_radiance_shape=(16, 64),
                            scatter_shape=(64, 128, 64, 16), ray_steps=128, sphere_steps=28, iterations=10)
     raise KeyError(name)
 

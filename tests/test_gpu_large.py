@@ -1,6 +1,6 @@
 """Workloads beyond the full 5-order one, stage by stage:
 
-* large: S 64x128x64x16 (8,388,608 entries), T 128x512, E 32x128, 128 ray steps, sphere steps 28 (256
+* large: S 64x128x64x16 (8,388,608 entries), T 128x512, E 16x64, 128 ray steps, sphere steps 28 (256
   directions, half sphere 2625), 10 scattering orders, 100 km atmosphere (NL x NA = 1024: four entries per
   thread in the 4D kernels, one sample per folded-slab chunk, dE read from global memory);
 * production (the reference's own parameters, atmosphere_lut.clj:20-62): S 32x127x32x8, T 64x255, E 16x63,

@@ -1,4 +1,4 @@
-"""The learned part on the large workload (S 64x128x64x16, T 128x512, E 32x128, 128 ray steps, 256 sphere
+"""The learned part on the large workload (S 64x128x64x16, T 128x512, E 16x64, 128 ray steps, 256 sphere
 directions, 10 orders): 64 atmospheres drawn from the seeded splitmix64 stream over the training ranges,
 built plain in batches of 16 and seeded one by one.
 
