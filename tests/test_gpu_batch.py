@@ -40,9 +40,13 @@ def _small(atm, iterations):
                 ray_steps=16, sphere_steps=10, iterations=iterations)
 
 
-@pytest.mark.parametrize("iterations", [0, 1, 3])
-def test_batch_equals_single_builds(atm, gpu, iterations):
-    Ps = atmospheres(atm, 5, seed=7 + iterations, **_small(atm, iterations))
+@pytest.mark.parametrize("iterations,slab", [(0, (7, 4)), (1, (7, 4)), (3, (7, 4)), (2, (24, 16)), (2, (32, 24))])
+def test_batch_equals_single_builds(atm, gpu, iterations, slab):
+    """Slabs of 28, 384 and 768 (sun elevation, heading) entries: one, two and four entries per ray-scatter
+    thread."""
+    shape = _small(atm, iterations)
+    shape["scatter_shape"] = (6, 11) + slab
+    Ps = atmospheres(atm, 5, seed=7 + iterations, **shape)
     for i, P in enumerate(Ps):  # albedo and intensity may differ between the atmospheres of a batch
         P.planet.brightness[:] = [0.1 + 0.1 * i] * 3
         P.intensity[:] = [1.0, 0.9 + 0.05 * i, 1.1]

@@ -173,6 +173,25 @@ def test_generate_luts_small(atm, gpu, oracle, iterations, brightness):
     assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
 
 
+@pytest.mark.parametrize("scatter_shape", [(4, 6, 24, 16), (3, 5, 32, 24)])
+def test_generate_luts_wide_slabs(atm, gpu, oracle, scatter_shape):
+    """Sun-elevation x heading slabs of 384 and 768 entries: the ray-scatter kernel's instantiations with two
+    and four entries per thread (one- and two-sample chunks), against the oracle's whole driver."""
+    P = atm.make_params(planet={"height": 100000.0, "brightness": (0.3,) * 3}, transmittance_shape=(10, 17),
+                        surface_radiance_shape=(6, 11), scatter_shape=scatter_shape, ray_steps=16, sphere_steps=10,
+                        iterations=2)
+    OP = oracle_params(oracle, P)
+    T, E, S, M = oracle.generate_atmosphere_luts(OP)
+    got = gpu.generate_atmosphere_luts(P)
+    assert_close("transmittance", got["transmittance"], T, TOL_TRANSMITTANCE)
+    assert_close("surface-radiance", got["surface-radiance"], E, TOL_SCATTER)
+    assert_close("ray-scatter", got["ray-scatter"], oracle.convert_4d_to_2d(S), TOL_SCATTER)
+    assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
+    gpu.build(P)  # the graph-replayed build returns the same tables
+    for which, name in enumerate(TABLES):
+        assert np.array_equal(gpu.fetch(P, which), got[name]), name
+
+
 TABLES = ["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]
 
 
