@@ -751,6 +751,63 @@ __device__ __forceinline__ float4 lds4(const float4* base, unsigned byte_offset)
   return *(const float4*)((const char*)base + byte_offset);
 }
 
+// Samples whose per-sample terms (the lookups' T and sun-coordinate terms, the fold's steps) are requested
+// together, before the barrier that precedes their chunk: one wait per group instead of one per sample.
+constexpr int kStepGroup = 4;
+
+// Per-sample terms of kStepGroup consecutive samples for the lookups, in scalar registers: T(x, p_i) and
+// (alpha_i, beta_i), requested together (before the barrier that precedes their chunk).
+struct SampleGroup {
+  float4 T[kStepGroup];
+  float2 ab[kStepGroup];
+};
+
+__device__ __forceinline__ void load_sample_group(const float4* __restrict__ gT, const float4* __restrict__ gRS, int i,
+                                                  int n, SampleGroup& g) {
+#pragma unroll
+  for (int q = 0; q < kStepGroup; ++q) {
+    const int j = min(i + q, n - 1);  // in bounds
+    g.T[q] = gT[j];
+    const float4 r = gRS[j];
+    g.ab[q] = make_float2(r.z, r.w);
+  }
+}
+
+// ray_scatter_samples over samples b .. min(b + kStepGroup, qn) - 1 of the chunk (b a multiple of
+// kStepGroup), their terms in g: the same operations in the same order.
+template <bool kColumn>
+__device__ __forceinline__ float3 ray_scatter_sample_group(const SunCoord& sc, const float4* s_slab,
+                                                           const SampleGroup& g, int b, int qn, float row_bytes,
+                                                           float col_bytes, int NLA, float vl, float xl, Lerp La,
+                                                           float3 acc) {
+  const unsigned rb = (unsigned)row_bytes;
+  const float slab_bytes = (float)(NLA * (int)sizeof(float4));
+  float q_bytes = fmaf((float)b, slab_bytes, col_bytes);  // byte offset of the entry's column in slab b (exact)
+#pragma unroll
+  for (int q = 0; q < kStepGroup; ++q, q_bytes += slab_bytes) {
+    if (b + q < qn) {
+      const float4 w = g.T[q];
+      const float c = sun_coord(sc, g.ab[q], xl, vl);
+      const float row = lower_cell(c, sc.top - 1.0f);
+      const float s = c - row;
+      const unsigned off = (unsigned)fmaf(row, row_bytes, q_bytes);
+      float3 v;
+      if (kColumn) {
+        v = lerp3(lds4(s_slab, off), lds4(s_slab, off + rb), s);
+      } else {
+        const unsigned d = (unsigned)(La.i1 - La.i0) * (unsigned)sizeof(float4);
+        const float3 u = lerp3(lds4(s_slab, off), lds4(s_slab, off + d), La.s);
+        const float3 t = lerp3(lds4(s_slab, off + rb), lds4(s_slab, off + rb + d), La.s);
+        v = make_float3(fmaf(s, t.x - u.x, u.x), fmaf(s, t.y - u.y, u.y), fmaf(s, t.z - u.z, u.z));
+      }
+      acc.x = fmaf(w.x, v.x, acc.x);
+      acc.y = fmaf(w.y, v.y, acc.y);
+      acc.z = fmaf(w.z, v.z, acc.z);
+    }
+  }
+  return acc;
+}
+
 template <bool kColumn>
 __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const float4* s_slab,
                                                       const float4* __restrict__ gT, const float4* __restrict__ gRS,
@@ -819,34 +876,53 @@ __device__ __forceinline__ float4 fold_value(const FoldSlots& f, const float4 w)
                      fmaf(w.w, d.z, fmaf(w.z, c.z, fmaf(w.y, b.z, w.x * a.z))), 0.f);
 }
 
+// Fold steps of kStepGroup consecutive samples, held in scalar registers: requested together, one wait
+// for all of them instead of one per sample.
+struct StepGroup {
+  FoldStep s[kStepGroup];
+};
+
+template <class Steps>
+__device__ __forceinline__ void load_step_group(const Steps fs, int i, int n, StepGroup& g) {
+#pragma unroll
+  for (int q = 0; q < kStepGroup; ++q) g.s[q] = fold_step(fs, min(i + q, n - 1));  // in bounds
+}
+
 // Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1):
 // slab[q][kk] = sum over slots of weight x dJ[node][kk].  fs: the ray's per-build fold steps (node offsets
 // and weights by slot; workgroup-uniform, read with scalar loads, so are the branches); prev: the previous
 // sample's node offsets.  With one entry per thread the slots carry over from sample to sample and a slot
-// is re-read only where its node changed.
+// is re-read only where its node changed; sg holds the steps of the chunk's first kStepGroup samples on
+// entry and those of the next chunk's on return (n: samples of the ray).
 template <int KPT, class Steps>
 __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const Steps fs,
                                                  float4* slab, int i0, int qn, int NLA, int t, int nt, FoldSlots& f,
-                                                 int4& prev) {
+                                                 int4& prev, StepGroup& sg, int n) {
   if (KPT == 1) {
     const int kk = t;
     // lanes past the table read the last entry and store the same bits as the last entry's lane to the
     // same place: no branch around the store, which would give the loop a path that skips the wait for
     // the slot loads and make the next sample's loads wait for them one by one
     const int kl = min(kk, NLA - 1);
-    FoldStep st = fold_step(fs, i0);
-    for (int q = 0; q < qn; ++q) {
-      // the next sample's step is requested before this sample's loads (its latency hides behind them)
-      const FoldStep nx = fold_step(fs, i0 + min(q + 1, qn - 1));
-      const int4 o = st.off;
-      if (o.x >= 0 && o.x != prev.x) f.v[0] = dJ[o.x + kl];
-      if (o.y >= 0 && o.y != prev.y) f.v[1] = dJ[o.y + kl];
-      if (o.z >= 0 && o.z != prev.z) f.v[2] = dJ[o.z + kl];
-      if (o.w >= 0 && o.w != prev.w) f.v[3] = dJ[o.w + kl];
-      prev = o;
-      slab[q * NLA + kl] = fold_value(f, st.wt);
-      st = nx;
+    // sg holds the steps of samples i0 .. i0 + 3 (requested before the barrier that precedes this chunk)
+    for (int b = 0; b < qn; b += kStepGroup) {
+      if (b > 0) load_step_group(fs, i0 + b, n, sg);
+#pragma unroll
+      for (int q = 0; q < kStepGroup; ++q) {
+        if (b + q < qn) {
+          const FoldStep st = sg.s[q];
+          const int4 o = st.off;
+          if (o.x >= 0 && o.x != prev.x) f.v[0] = dJ[o.x + kl];
+          if (o.y >= 0 && o.y != prev.y) f.v[1] = dJ[o.y + kl];
+          if (o.z >= 0 && o.z != prev.z) f.v[2] = dJ[o.z + kl];
+          if (o.w >= 0 && o.w != prev.w) f.v[3] = dJ[o.w + kl];
+          prev = o;
+          slab[(b + q) * NLA + kl] = fold_value(f, st.wt);
+        }
+      }
     }
+    // the next chunk's first steps, in flight across the barrier
+    if (i0 + qn < n) load_step_group(fs, i0 + qn, n, sg);
   } else {  // KPT (even) entries per thread, NLA <= KPT nt: every sample reads its four nodes
     for (int q = 0; q < qn; ++q) {
       const FoldStep st = fold_step(fs, i0 + q);
@@ -951,11 +1027,13 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     FoldSlots fs;
     fold_slots_init(fs);
     int4 prev = make_int4(-1, -1, -1, -1);
-    ray_scatter_fold<EPT>(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, prev);
+    StepGroup sg;
+    if constexpr (EPT == 1) load_step_group(gF, 0, n, sg);
+    ray_scatter_fold<EPT>(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, prev, sg, n);
     __syncthreads();
     for (int c = 0; c + 1 < chunks; ++c) {
       ray_scatter_fold<EPT>(dJ, gF, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
-                              t, kRSHalf, fs, prev);
+                            t, kRSHalf, fs, prev, sg, n);
       __syncthreads();
     }
     if (kFoldResamples && accum_arg.S_out) {
@@ -996,22 +1074,40 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
       La[q] = make_lerp(m.z, NA);
     }
   }
+  SampleGroup sgp;
+  if constexpr (EPT == 1) load_sample_group(gT, gRS, 0, n, sgp);
   __syncthreads();  // chunk 0 folded
   for (int c = 0; c < chunks; ++c) {
     const int i0 = c * Q, qn = min(Q, n - i0);
     const float4* slab = s_slab + (c & 1) * Q * NLA;
+    if constexpr (EPT == 1) {
+      for (int b = 0; b < qn; b += kStepGroup) {
+        if (b > 0) load_sample_group(gT, gRS, i0 + b, n, sgp);
+        if (t < NLA) {
+          const float col_bytes = (float)(La[0].i0 * (int)sizeof(float4));
+          // a heading coordinate with zero weight (every entry but the clamped ones) reads one column;
+          // the choice is made per wavefront so that a wavefront never runs both loops
+          if (__all(La[0].s == 0.0f))
+            acc[0] = ray_scatter_sample_group<true>(sc, slab, sgp, b, qn, row_bytes, col_bytes, NLA, vl[0], xl[0],
+                                                    La[0], acc[0]);
+          else
+            acc[0] = ray_scatter_sample_group<false>(sc, slab, sgp, b, qn, row_bytes, col_bytes, NLA, vl[0], xl[0],
+                                                     La[0], acc[0]);
+        }
+      }
+      if (c + 1 < chunks) load_sample_group(gT, gRS, i0 + qn, n, sgp);  // in flight across the barrier
+    } else {  // several entries per lane: the terms per sample (a group would spill)
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      if (t + e * kRSHalf < NLA) {
-        const float col_bytes = (float)(La[e].i0 * (int)sizeof(float4));
-        // a heading coordinate with zero weight (every entry but the clamped ones) reads one column;
-        // the choice is made per wavefront so that a wavefront never runs both loops
-        if (__all(La[e].s == 0.0f))
-          acc[e] = ray_scatter_samples<true>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
-                                             La[e], acc[e]);
-        else
-          acc[e] = ray_scatter_samples<false>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
-                                              La[e], acc[e]);
+      for (int e = 0; e < EPT; ++e) {
+        if (t + e * kRSHalf < NLA) {
+          const float col_bytes = (float)(La[e].i0 * (int)sizeof(float4));
+          if (__all(La[e].s == 0.0f))
+            acc[e] = ray_scatter_samples<true>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
+                                               La[e], acc[e]);
+          else
+            acc[e] = ray_scatter_samples<false>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
+                                                La[e], acc[e]);
+        }
       }
     }
     __syncthreads();
