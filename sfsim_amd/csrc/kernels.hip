@@ -887,34 +887,51 @@ __device__ __forceinline__ void load_step_group(const Steps fs, int i, int n, St
 
 // Fold of the chunk of samples i0 .. i0 + qn - 1 into `slab` by the nt folding threads (t = 0 .. nt - 1):
 // slab[q][kk] = sum over slots of weight x dJ[node][kk].  fs: the ray's per-build fold steps (node offsets
-// and weights by slot; workgroup-uniform, read with scalar loads, so are the branches); prev: the previous
-// sample's node offsets.  With one entry per thread the slots carry over from sample to sample and a slot
-// is re-read only where its node changed; sg holds the steps of the chunk's first kStepGroup samples on
-// entry and those of the next chunk's on return (n: samples of the ray).
+// and weights by slot; workgroup-uniform, read with scalar loads, so are the branches).  KPT == 1 (the
+// kernel with one entry per lookup lane, NLA <= nt = 256): the slots (f, f2) carry over from one of the
+// thread's samples to the next and a slot is re-read only where its node changed; prev: the node offsets of
+// the thread's previous sample; sg holds the steps of the chunk's first kStepGroup samples on entry and
+// those of the next chunk's on return (n: samples of the ray).  KPT > 1: every sample reads its nodes.
 template <int KPT, class Steps>
 __device__ __forceinline__ void ray_scatter_fold(const float4* __restrict__ dJ, const Steps fs,
                                                  float4* slab, int i0, int qn, int NLA, int t, int nt, FoldSlots& f,
-                                                 int4& prev, StepGroup& sg, int n) {
+                                                 FoldSlots& f2, int4& prev, StepGroup& sg, int n) {
   if (KPT == 1) {
-    const int kk = t;
-    // lanes past the table read the last entry and store the same bits as the last entry's lane to the
-    // same place: no branch around the store, which would give the loop a path that skips the wait for
-    // the slot loads and make the next sample's loads wait for them one by one
-    const int kl = min(kk, NLA - 1);
+    // the four fold wavefronts split the ray's samples by parity, which halves the chain of dependent
+    // node loads that each of them walks: wavefronts 0-1 fold the even samples, 2-3 the odd ones, each lane
+    // two entries of its pair's 256 (f: the first, f2: the second), the slots carrying over from the
+    // wavefront's previous sample, two samples back.  Lanes past the table read the last entry and store
+    // the same bits as the last entry's lane to the same place (no branch around the store, which would
+    // give the loop a path that skips the wait for the slot loads).
+    const int w = t >> 6, lane = t & 63, par = w >> 1;
+    const int kl0 = min((w & 1) * 128 + lane, NLA - 1), kl1 = min((w & 1) * 128 + 64 + lane, NLA - 1);
     // sg holds the steps of samples i0 .. i0 + 3 (requested before the barrier that precedes this chunk)
     for (int b = 0; b < qn; b += kStepGroup) {
       if (b > 0) load_step_group(fs, i0 + b, n, sg);
 #pragma unroll
       for (int q = 0; q < kStepGroup; ++q) {
-        if (b + q < qn) {
+        if (b + q < qn && ((i0 + b + q) & 1) == par) {
           const FoldStep st = sg.s[q];
           const int4 o = st.off;
-          if (o.x >= 0 && o.x != prev.x) f.v[0] = dJ[o.x + kl];
-          if (o.y >= 0 && o.y != prev.y) f.v[1] = dJ[o.y + kl];
-          if (o.z >= 0 && o.z != prev.z) f.v[2] = dJ[o.z + kl];
-          if (o.w >= 0 && o.w != prev.w) f.v[3] = dJ[o.w + kl];
+          if (o.x >= 0 && o.x != prev.x) {
+            f.v[0] = dJ[o.x + kl0];
+            f2.v[0] = dJ[o.x + kl1];
+          }
+          if (o.y >= 0 && o.y != prev.y) {
+            f.v[1] = dJ[o.y + kl0];
+            f2.v[1] = dJ[o.y + kl1];
+          }
+          if (o.z >= 0 && o.z != prev.z) {
+            f.v[2] = dJ[o.z + kl0];
+            f2.v[2] = dJ[o.z + kl1];
+          }
+          if (o.w >= 0 && o.w != prev.w) {
+            f.v[3] = dJ[o.w + kl0];
+            f2.v[3] = dJ[o.w + kl1];
+          }
           prev = o;
-          slab[(b + q) * NLA + kl] = fold_value(f, st.wt);
+          slab[(b + q) * NLA + kl0] = fold_value(f, st.wt);
+          slab[(b + q) * NLA + kl1] = fold_value(f2, st.wt);
         }
       }
     }
@@ -1021,16 +1038,17 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
       if constexpr (kB) return (ConstFoldSteps)gS;
       else return gS;
     }();
-    FoldSlots fs;
+    FoldSlots fs, fs2;
     fold_slots_init(fs);
+    fold_slots_init(fs2);
     int4 prev = make_int4(-1, -1, -1, -1);
     StepGroup sg;
     if constexpr (EPT == 1) load_step_group(gF, 0, n, sg);
-    ray_scatter_fold<EPT>(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, prev, sg, n);
+    ray_scatter_fold<EPT>(dJ, gF, s_slab, 0, min(Q, n), NLA, t, kRSHalf, fs, fs2, prev, sg, n);
     __syncthreads();
     for (int c = 0; c + 1 < chunks; ++c) {
       ray_scatter_fold<EPT>(dJ, gF, s_slab + ((c + 1) & 1) * Q * NLA, (c + 1) * Q, min(Q, n - (c + 1) * Q), NLA,
-                            t, kRSHalf, fs, prev, sg, n);
+                            t, kRSHalf, fs, fs2, prev, sg, n);
       __syncthreads();
     }
     if (kFoldResamples && accum_arg.S_out) {
