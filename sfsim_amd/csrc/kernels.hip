@@ -616,7 +616,9 @@ __device__ __forceinline__ void ray_geometry_body(const DevParams& P, WorkCounte
   }
   if (P.sun_rows > 0) {
     const int cols = P.sun_cols;
-    const double rho = sqrt(h * (2.0 * P.g.R + h));
+    // clamped like horizon_distance: the sample of a ray that grazes the ground at its tangent point (the
+    // middle sample of an odd number of them) has h = 0 up to FP64 rounding, of either sign
+    const double rho = sqrt(fmax(0.0, h * (2.0 * P.g.R + h)));
     const double dmin = P.g.H - h, dmax = rho + P.g.rho_max;
     V.ss[g] = make_float4((float)((P.g.H - h) * (2.0 * P.g.R + P.g.H + h)), (float)dmin,
                           (float)((cols - 1) / (dmax - dmin)), (float)exp(-h * P.inv_scale_ray));

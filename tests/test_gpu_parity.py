@@ -192,6 +192,27 @@ def test_generate_luts_wide_slabs(atm, gpu, oracle, scatter_shape):
         assert np.array_equal(gpu.fetch(P, which), got[name]), name
 
 
+@pytest.mark.parametrize("scatter_shape,ray_steps", [((4, 6, 32, 8), 13), ((4, 6, 7, 5), 11)])
+def test_generate_luts_odd_ray_steps(atm, gpu, oracle, scatter_shape, ray_steps):
+    """Odd numbers of view samples for the one-entry-per-lane ray-scatter kernel, whose fold wavefronts split
+    the samples by parity: 13 samples in chunks of four (the last chunk one even sample) over a 256-entry
+    slab, and 11 samples in chunks of eight (a second group of steps inside a chunk, and lanes past the
+    35-entry slab), against the oracle's whole driver; the graph-replayed build returns the same tables."""
+    P = atm.make_params(planet={"height": 100000.0, "brightness": (0.3,) * 3}, transmittance_shape=(10, 17),
+                        surface_radiance_shape=(6, 11), scatter_shape=scatter_shape, ray_steps=ray_steps,
+                        sphere_steps=10, iterations=2)
+    OP = oracle_params(oracle, P)
+    T, E, S, M = oracle.generate_atmosphere_luts(OP)
+    got = gpu.generate_atmosphere_luts(P)
+    assert_close("transmittance", got["transmittance"], T, TOL_TRANSMITTANCE)
+    assert_close("surface-radiance", got["surface-radiance"], E, TOL_SCATTER)
+    assert_close("ray-scatter", got["ray-scatter"], oracle.convert_4d_to_2d(S), TOL_SCATTER)
+    assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
+    gpu.build(P)
+    for which, name in enumerate(TABLES):
+        assert np.array_equal(gpu.fetch(P, which), got[name]), name
+
+
 TABLES = ["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]
 
 
