@@ -990,11 +990,13 @@ constexpr int kRSHalf = kRSBlock / 2;
 template <int EPT, bool kB>
 __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) ray_scatter_kernel(
     const DevParams P0, const DevParams* __restrict__ PB, const float4* __restrict__ dJ,
-    const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs, const FoldStep* __restrict__ geo_fold,
-    float4* __restrict__ out, const RsAccum accum_arg, int Q, WorkCounters* wc) {
+    const double2* __restrict__ geo_th, const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs,
+    const FoldStep* __restrict__ geo_fold, float4* __restrict__ out, const RsAccum accum_arg, int Q,
+    WorkCounters* wc) {
   const DevParams& P = params_of<kB>(P0, PB);
   if constexpr (kB) {  // this atmosphere's tables and ray geometry (the accumulation tables where they are used);
                        // offsets of the restrict arguments, whose loads stay provably unclobbered and uniform
+    geo_th = atm_ptr<kB>(geo_th, P0);
     geo_T = atm_ptr<kB>(geo_T, P0);
     geo_rs = atm_ptr<kB>(geo_rs, P0);
     // the fold steps depend only on the planet and the grid, which the atmospheres of a batch share: all
@@ -1023,7 +1025,6 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
   // given a contiguous range of impact parameters reads a band of the dJ table
   const int ord = xcd_ray(blockIdx.x, NH * NE);
   const int rayid = P.ray_order ? P.ray_order[ord] : ord;
-  const int ih = rayid / NE, ie = rayid - ih * NE;
   const bool folder = threadIdx.x < kRSHalf;  // whole wavefronts
   const int t = folder ? (int)threadIdx.x : (int)threadIdx.x - kRSHalf;
   // this ray's samples in the per-build geometry: T(x, p_i) rgb and (height coordinate, elevation
@@ -1068,8 +1069,8 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     __syncthreads();
     return;
   }
-  const Ray ray = make_ray(P, ih, ie);
-  const double dt = ray.D / n;
+  // the ray's step D / n: twice its first sample's t = 0.5 D / n (FP64, exact), from the per-build geometry
+  const double dt = 2.0 * geo_th[(long long)rayid * n].x;
   const SunCoord sc = {(float)((NL - 1) * (double)P.sun_inv_den), (float)(-0.6 * 1.4426950408889634), (float)(NL - 1)};
   const float row_bytes = (float)(NA * (int)sizeof(float4));
   // per-entry constants: heading coordinate and the terms of dot(p_i, l)
@@ -2079,24 +2080,24 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   if (P.batch > 0) {
     const dim3 grid(rays, 1, (unsigned)P.batch);
     if (nla <= kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<1, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         V.fold, out, acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<1, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
+                         V.rs, V.fold, out, acc, Q, wc);
     else if (nla <= 2 * kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<2, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         V.fold, out, acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<2, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
+                         V.rs, V.fold, out, acc, Q, wc);
     else
-      hipLaunchKernelGGL((ray_scatter_kernel<4, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         V.fold, out, acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<4, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
+                         V.rs, V.fold, out, acc, Q, wc);
   } else {
     if (nla <= kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<1, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         V.fold, out, acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<1, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
+                         V.rs, V.fold, out, acc, Q, wc);
     else if (nla <= 2 * kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<2, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         V.fold, out, acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<2, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
+                         V.rs, V.fold, out, acc, Q, wc);
     else
-      hipLaunchKernelGGL((ray_scatter_kernel<4, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.T, V.rs,
-                         V.fold, out, acc, Q, wc);
+      hipLaunchKernelGGL((ray_scatter_kernel<4, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
+                         V.rs, V.fold, out, acc, Q, wc);
   }
   return hipGetLastError();
 }
