@@ -1573,6 +1573,7 @@ __global__ void __launch_bounds__(kBlock) surface_radiance_kernel(const DevParam
 __device__ __forceinline__ void resample4_lerps(const DevParams& P, int ih, int ie, int is, int ia, Lerp& Lh, Lerp& Le,
                                                 Lerp& Ls, Lerp& La, float4* ps = nullptr, float4* rs = nullptr,
                                                 double2* ss = nullptr) {
+  SFSIM_EXACT  // the reference's rounding (see geometry.hpp)
   const int NH = P.nS[0], NE = P.nS[1], NL = P.nS[2], NA = P.nS[3];
   double r0 = index_to_radius(P.g, NH, ih);
   double se, ce;
@@ -1750,6 +1751,7 @@ __device__ float3 resample4_value_lean(const DevParams& P, const float4* __restr
 // surface-radiance (space 3) table.
 __device__ float3 resample2_value(const DevParams& P, int space, const float4* __restrict__ a,
                                  const float4* __restrict__ b, long long idx) {
+  SFSIM_EXACT  // the reference's rounding (see geometry.hpp)
   const int N0 = space == 2 ? P.nT[0] : P.nE[0], N1 = space == 2 ? P.nT[1] : P.nE[1];
   int i0 = (int)(idx / N1), i1 = (int)(idx % N1);
   double r0 = index_to_radius(P.g, N0, i0);

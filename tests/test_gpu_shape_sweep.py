@@ -1,0 +1,90 @@
+"""Seeded sweep over small grid shapes, sample counts, quadratures, iteration counts and atmosphere heights
+(generate-atmosphere-luts, atmosphere_lut.clj:43-105): every table of the GPU's whole driver against the
+oracle's at the parity tolerances.
+
+The reference's tests fix a handful of shapes; this sweep takes sizes from the smallest the interface
+accepts (2 per dimension, 1 view sample) upward, odd and even, so that chunking, the per-lane entry
+split, the parity split of the fold and the horizon rows meet grids that no hand-picked case covers.
+It found the grazing ray's NaN at odd sample counts and the fused multiply-adds in the FP64 index round
+trips (coarse transmittance grids, up to 7e-3), both fixed; one case stays a documented FP32 limit.
+"""
+import numpy as np
+import pytest
+
+from tests.parity import TOL_SCATTER, TOL_TRANSMITTANCE, assert_close, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def sweep_cases(count=40, seed=7):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(count):
+        iterations = int(rng.integers(0, 4))
+        cases.append(dict(
+            height=float(rng.choice([35000.0, 60000.0, 100000.0])),
+            transmittance_shape=(int(rng.integers(2, 17)), int(rng.integers(2, 33))),
+            surface_radiance_shape=(int(rng.integers(2, 9)), int(rng.integers(2, 17))),
+            scatter_shape=(int(rng.integers(2, 10)), int(rng.integers(2, 15)), int(rng.integers(2, 13)),
+                           int(rng.integers(2, 13))),
+            # the surface-radiance half sphere takes ray_steps >> 2 rings: at least 4 samples with iterations
+            ray_steps=int(rng.integers(4 if iterations else 1, 41)),
+            sphere_steps=int(rng.integers(2, 15)),
+            iterations=iterations))
+    return cases
+
+
+@pytest.fixture(scope="module")
+def atm():
+    from sfsim_amd import atmosphere
+    return atmosphere
+
+
+@pytest.fixture(scope="module")
+def gpu(atm):
+    ctx = atm.Atmosphere(0)
+    yield ctx
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    from oracle import atmo
+    return atmo
+
+
+def case_id(c):
+    return "x".join(map(str, c["scatter_shape"])) + f"-n{c['ray_steps']}-s{c['sphere_steps']}-it{c['iterations']}"
+
+
+# The ray-scatter lookup evaluates the sun-elevation coordinate in FP32 as A - A exp2(arg).  Where a view
+# ray is vertical and the sun sits at the bottom of the index range (sin = -0.2, where the coordinate is
+# clamped at 0 and arg cancels to 0), the coordinate carries ~1e-6 index units of rounding, a weight that
+# the FP64 oracle does not have on the next sun-elevation row; on a 12-row grid that row is orders of
+# magnitude brighter, and an entry at 2e-6 of the table's maximum (just above the comparison floor) is off
+# by 1.6e-3.  Production grids (32 rows) stay well inside the bound.
+FP32_SUN_COORDINATE = {"2x4x12x9-n36-s4-it1"}
+
+
+def marked(cases):
+    out = []
+    for c in cases:
+        marks = [pytest.mark.xfail(strict=True, reason="FP32 sun-elevation coordinate at the index clamp, "
+                                   "12-row grid")] if case_id(c) in FP32_SUN_COORDINATE else []
+        out.append(pytest.param(c, id=case_id(c), marks=marks))
+    return out
+
+
+@pytest.mark.parametrize("case", marked(sweep_cases()))
+def test_sweep_against_oracle(atm, gpu, oracle, case):
+    c = dict(case)
+    P = atm.make_params(planet={"height": c.pop("height"), "brightness": (0.3,) * 3}, **c)
+    T, E, S, M = oracle.generate_atmosphere_luts(oracle_params(oracle, P))
+    got = gpu.generate_atmosphere_luts(P)
+    assert_close("transmittance", got["transmittance"], T, TOL_TRANSMITTANCE)
+    if P.iterations:
+        assert_close("surface-radiance", got["surface-radiance"], E, TOL_SCATTER)
+    else:
+        assert not got["surface-radiance"].any()
+    assert_close("ray-scatter", got["ray-scatter"], oracle.convert_4d_to_2d(S), TOL_SCATTER)
+    assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
