@@ -88,3 +88,25 @@ def test_sweep_against_oracle(atm, gpu, oracle, case):
         assert not got["surface-radiance"].any()
     assert_close("ray-scatter", got["ray-scatter"], oracle.convert_4d_to_2d(S), TOL_SCATTER)
     assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
+
+
+TABLES = ["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=case_id(c)) for c in sweep_cases() if c["iterations"]][:10])
+def test_sweep_batch_equals_single_builds(atm, gpu, case):
+    """The batched build (one launch sequence over three atmospheres of different scattering coefficients,
+    albedos and intensities) equals single builds bit for bit at the sweep's shapes."""
+    from tests.test_atmosphere_batch import atmospheres
+    c = dict(case)
+    height = c.pop("height")
+    Ps = atmospheres(atm, 3, seed=11, **c)
+    for i, P in enumerate(Ps):
+        P.planet.height = height
+        P.planet.brightness[:] = [0.15 + 0.1 * i] * 3
+        P.intensity[:] = [1.0, 0.95 + 0.05 * i, 1.05]
+    gpu.build_batch(Ps)
+    for a, P in enumerate(Ps):
+        single = gpu.generate_atmosphere_luts(P)
+        for which, name in enumerate(TABLES):
+            assert np.array_equal(gpu.fetch_batch(P, a, which), single[name]), (a, name)

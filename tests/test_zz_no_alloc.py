@@ -64,6 +64,26 @@ def test_noalloc_build_equals_generate_luts():
                 bufs.free()
 
 
+def test_noalloc_sweep_shapes():
+    """The same equality at ten of the shape sweep's configurations, called one after another on one
+    context (each call on a grid of a different shape than the last)."""
+    from sfsim_amd import atmosphere as atm
+    from tests.test_gpu_shape_sweep import sweep_cases
+    with atm.Atmosphere(0) as ctx:
+        for case in sweep_cases()[:10]:
+            c = dict(case)
+            P = atm.make_params(planet={"height": c.pop("height"), "brightness": (0.3,) * 3}, **c)
+            print("noalloc sweep", tuple(P.scatter_shape), P.ray_steps, P.iterations, flush=True)
+            ref = ctx.generate_atmosphere_luts(P)
+            bufs = DeviceBuffers([ref[k].size for k in TABLES])
+            try:
+                ctx.generate_luts_noalloc(P, bufs.ptrs)
+                for i, k in enumerate(TABLES):
+                    assert np.array_equal(bufs.read(i), ref[k].reshape(-1)), (tuple(P.scatter_shape), k)
+            finally:
+                bufs.free()
+
+
 def test_noalloc_rejects_what_does_not_fit():
     from sfsim_amd import atmosphere as atm
     with atm.Atmosphere(0) as ctx:
