@@ -701,16 +701,16 @@ int enqueue_pipeline(sfsim_atmo_ctx* ctx, const sfsim_atmo_params* p, const DevP
   // (atmosphere_lut.clj:97-105), the independent ones in one launch: T, E and M1 beside S <- table(S + dS)
   ResampleJob jobs[kMaxResampleJobs];
   int nj = 0;
-  jobs[nj++] = ResampleJob{2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0};
-  jobs[nj++] = ResampleJob{4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1};
-  if (E_cur) jobs[nj++] = ResampleJob{3, E_cur, nullptr, nullptr, (float*)ctx->out[1].ptr, 0};
+  jobs[nj++] = ResampleJob{2, f4(ctx->T), nullptr, nullptr, (float*)ctx->out[0].ptr, 0, nullptr};
+  jobs[nj++] = ResampleJob{4, f4(ctx->M1), nullptr, nullptr, (float*)ctx->out[3].ptr, 1, nullptr};
+  if (E_cur) jobs[nj++] = ResampleJob{3, E_cur, nullptr, nullptr, (float*)ctx->out[1].ptr, 0, nullptr};
   if (S_resample) {  // S_in: the last iteration's S + dS
-    jobs[nj++] = ResampleJob{4, S_in, nullptr, f4(ctx->Sacc[0]), nullptr, 0};
+    jobs[nj++] = ResampleJob{4, S_in, nullptr, f4(ctx->Sacc[0]), nullptr, 0, nullptr};
     LAUNCH(timer, kFinal, launch_resamples(P, jobs, nj, ctx->stream));
     LAUNCH(timer, kFinal, launch_resample(P, 4, f4(ctx->Sacc[0]), nullptr, nullptr, (float*)ctx->out[2].ptr, 1,
                                           ctx->stream));
   } else {  // S = R1
-    jobs[nj++] = ResampleJob{4, S_in, nullptr, nullptr, (float*)ctx->out[2].ptr, 1};
+    jobs[nj++] = ResampleJob{4, S_in, nullptr, nullptr, (float*)ctx->out[2].ptr, 1, nullptr};
     LAUNCH(timer, kFinal, launch_resamples(P, jobs, nj, ctx->stream));
   }
   if (!E_cur)  // every atmosphere's surface-radiance table (blocks atm_stride apart in a batch)

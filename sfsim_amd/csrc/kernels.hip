@@ -25,8 +25,6 @@ constexpr int kBlock = 256;
 constexpr int kMaxRayChunk = 8;  // samples per slab chunk of the ray-scatter kernel
 
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // min(floor(x), hi) for a coordinate already clamped to x >= 0, as one med3 (fminf would quiet a
 // signalling NaN in hi on every call)
 __device__ __forceinline__ float lower_cell(float x, float hi) { return __builtin_amdgcn_fmed3f(floorf(x), hi, 0.0f); }
@@ -73,12 +71,13 @@ struct Segment {
   bool graze;           // lowest point interior and below the FP64 altitude threshold
 };
 
-__device__ __forceinline__ Segment make_segment(const DevParams& P, double h0, double pd0, double D, int n) {
+// The midpoint nodes of a segment of length D in P.ray_steps steps (every segment of the path has that many).
+__device__ __forceinline__ Segment make_segment(const DevParams& P, double h0, double pd0, double D) {
   Segment g;
   double ts = fmin(fmax(-pd0, 0.0), D);
   g.c0d = h0 * (2.0 * P.g.R + h0) + ts * (2.0 * pd0 + ts);
   g.bd = 2.0 * pd0 + 2.0 * ts;
-  g.dtd = D * P.inv_steps;  // n == P.ray_steps at every call
+  g.dtd = D * P.inv_steps;
   g.s0d = 0.5 * g.dtd - ts;
   g.c0 = (float)(g.c0d * P.inv_R * P.inv_R);
   g.b = (float)(g.bd * P.inv_R);
@@ -184,7 +183,7 @@ __device__ __forceinline__ float3 segment_transmittance_f64(const DevParams& P, 
 // chosen per wavefront (when any active lane holds a grazing segment) so that the two loops never
 // diverge within a wave.
 __device__ float3 segment_transmittance(const DevParams& P, double h0, double pd0, double D, int n) {
-  Segment g = make_segment(P, h0, pd0, D, n);
+  Segment g = make_segment(P, h0, pd0, D);
   if (__any(g.graze)) return segment_transmittance_f64(P, g, D, n);
   return segment_transmittance_f32(P, g, D, n);
 }
@@ -286,11 +285,6 @@ __device__ __forceinline__ float3 lookup4_skip(Ld ld, int N1, int N2, int N3, Le
   };
   const float3 r0 = side(a.i0);
   return a.s == 0.0f ? r0 : mix3(r0, side(a.i1), a.s);
-}
-
-// atmosphere.clj:322-326 in FP32
-__device__ __forceinline__ float sun_elevation_coord(const DevParams& P, int size, float sin_elevation) {
-  return (size - 1) * fmaxf(0.0f, (1.0f - __expf(-3.0f * sin_elevation - 0.6f)) * P.sun_inv_den);
 }
 
 __device__ __forceinline__ void count(WorkCounters* wc, unsigned long long single, unsigned long long samples,
@@ -487,7 +481,7 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
       const double h = s_h[i];
       const double pl = s_xl[owner] + s_t[i] * s_vl[owner];  // dot(p_i, l)
       const double Ds = sun_exit_distance(P.g, h, pl);
-      const float3 Ts = segment_transmittance_f32(P, make_segment(P, h, pl, Ds, n), Ds, n);
+      const float3 Ts = segment_transmittance_f32(P, make_segment(P, h, pl, Ds), Ds, n);
       s_res[it] = Ts.x;
       s_res[it + kSSWave * kSSChunk] = Ts.y;
       s_res[it + 2 * kSSWave * kSSChunk] = Ts.z;
@@ -498,7 +492,7 @@ __global__ void __launch_bounds__(kSSWave) single_scatter_kernel(const DevParams
       const double h = s_h[i];
       const double pl = s_xl[owner] + s_t[i] * s_vl[owner];
       const double Ds = sun_exit_distance(P.g, h, pl);
-      const float3 Ts = segment_transmittance_f64(P, make_segment(P, h, pl, Ds, n), Ds, n);
+      const float3 Ts = segment_transmittance_f64(P, make_segment(P, h, pl, Ds), Ds, n);
       s_res[it] = Ts.x;
       s_res[it + kSSWave * kSSChunk] = Ts.y;
       s_res[it + 2 * kSSWave * kSSChunk] = Ts.z;
@@ -559,7 +553,7 @@ __device__ __forceinline__ void sun_depth_body(const DevParams& P, unsigned blk)
   float2 out = make_float2(-256.f, -256.f);  // empty segment (the top of the atmosphere, sun at the zenith)
   if (D > 0.0) {
     const double pl = (c - D * D) / (2.0 * D);  // from D = -pl + sqrt(pl^2 + c)
-    const Segment g = make_segment(P, h, pl, D, P.ray_steps);
+    const Segment g = make_segment(P, h, pl, D);
     double sm, sr;
     if (__any(g.graze)) {  // the same FP32/FP64 choice as segment_transmittance
       const double2 sum = density_sums_f64(P, g, P.ray_steps);
