@@ -1265,8 +1265,18 @@ __device__ __forceinline__ float mie_phase(const PsConst& pc, float mu) {
   return pc.mie_k * fmaf(mu, mu, 1.0f) * (y * y * y);
 }
 
-__device__ __forceinline__ float4 ld4b(const float4* base, unsigned byte_offset) {
-  return *(const float4*)((const char*)base + byte_offset);
+// the texel at byte_offset and the next one (the pointer is formed once, so the second read takes the
+// first one's address register with an immediate offset)
+__device__ __forceinline__ void ld4b_pair(const float4* base, unsigned byte_offset, float4& c0, float4& c1) {
+  const float4* p = (const float4*)((const char*)base + byte_offset);
+  c0 = p[0];
+  c1 = p[1];
+}
+
+__device__ __forceinline__ float3 lerp_pair(const float4* base, unsigned byte_offset, float s) {
+  float4 c0, c1;
+  ld4b_pair(base, byte_offset, c0, c1);
+  return lerp3(c0, c1, s);
 }
 
 // Sum over the quadrature directions of one entry (sun direction l, sun-elevation lerp Ls; row_bytes:
@@ -1295,14 +1305,14 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
     const float4* row = sh + j * NLA;
     float3 v = lerp3(c0, c1, sa);
     if (!kRow) {
-      const float3 u = lerp3(ld4b(row, off + d_row), ld4b(row, off + d_row + 16u), sa);
+      const float3 u = lerp_pair(row, off + d_row, sa);
       v = make_float3(fmaf(Ls.s, u.x - v.x, v.x), fmaf(Ls.s, u.y - v.y, v.y), fmaf(Ls.s, u.z - v.z, v.z));
     }
     if (kMie) {
       const float4* rowm = shm + j * NLA;
-      float3 m = lerp3(ld4b(rowm, off), ld4b(rowm, off + 16u), sa);
+      float3 m = lerp_pair(rowm, off, sa);
       if (!kRow) {
-        const float3 u = lerp3(ld4b(rowm, off + d_row), ld4b(rowm, off + d_row + 16u), sa);
+        const float3 u = lerp_pair(rowm, off + d_row, sa);
         m = make_float3(fmaf(Ls.s, u.x - m.x, m.x), fmaf(Ls.s, u.y - m.y, m.y), fmaf(Ls.s, u.z - m.z, m.z));
       }
       const float ph = mie_phase(pc, ol);
@@ -1317,7 +1327,7 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
       const float ecol = lower_cell(ce, pc.se.top - 1.0f);
       const float se = ce - ecol;
       const unsigned eo = (unsigned)ecol * (unsigned)sizeof(float4);  // height row 0: P is on the ground
-      const float3 u = lerp3(ld4b(dE, eo), ld4b(dE, eo + 16u), se);
+      const float3 u = lerp_pair(dE, eo, se);
       v.x = fmaf(dd.ground.x, u.x, v.x);
       v.y = fmaf(dd.ground.y, u.y, v.y);
       v.z = fmaf(dd.ground.z, u.z, v.z);
@@ -1337,8 +1347,7 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
     for (int d = 0; d < kB; ++d) {
       off[d] = heading(j + d, ol[d], sa[d]);
       const float4* row = sh + (j + d) * NLA;
-      c0[d] = ld4b(row, off[d]);
-      c1[d] = ld4b(row, off[d] + 16u);
+      ld4b_pair(row, off[d], c0[d], c1[d]);
     }
 #pragma unroll
     for (int d = 0; d < kB; ++d) term(j + d, ol[d], sa[d], off[d], c0[d], c1[d]);
@@ -1347,7 +1356,9 @@ __device__ __forceinline__ float3 point_scatter_dirs(const PsConst& pc, const Di
     float ol, sa;
     const unsigned off = heading(j, ol, sa);
     const float4* row = sh + j * NLA;
-    term(j, ol, sa, off, ld4b(row, off), ld4b(row, off + 16u));
+    float4 c0, c1;
+    ld4b_pair(row, off, c0, c1);
+    term(j, ol, sa, off, c0, c1);
   }
   return acc;
 }
