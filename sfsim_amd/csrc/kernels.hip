@@ -736,6 +736,26 @@ __device__ __forceinline__ float sun_coord(const SunCoord& sc, float2 ab, float 
   return __builtin_amdgcn_fmed3f(fmaf(-sc.A, e, sc.A), 0.0f, sc.top);
 }
 
+// Sun-elevation grids with fewer rows than this take the exact form below.  Their rows lie far apart in
+// sun elevation, so the row above a sun just below the index range's lower end (sin = -0.2, where the
+// coordinate is clamped at 0) can be brighter by orders of magnitude, and sun_coord's FP32 exponent,
+// which cancels to 0 there (rounding ~3e-7 index units), would weight that row by as much.
+constexpr int kExactSunRows = 16;
+
+// sun_coord with the exponent x = log2(e) (-3 dot(p_i, l) / r_i - 0.6) formed in FP64 from the sample's
+// t_i, h_i and the entry's dot(x, l), dot(v, l) (1 / r_i: the FP32 estimate and one FP64 Newton step,
+// relative error ~1e-14), and 1 - 2^x from its series where |x| < 1/32 (no cancellation near the clamp).
+__device__ __forceinline__ float sun_coord_exact(const SunCoord& sc, double R, double2 th, double xl, double vl) {
+  const double r = R + th.y;
+  double inv = (double)__builtin_amdgcn_rcpf((float)r);
+  inv = fma(inv, fma(-r, inv, 1.0), inv);
+  const double a = -3.0 * 1.4426950408889634 * inv;
+  const float x = (float)fma(a * th.x, vl, fma(a, xl, -0.6 * 1.4426950408889634));
+  const float y = x * 0.6931471805599453f;
+  const float om = fabsf(x) < 0.03125f ? -y * fmaf(y, fmaf(y, 1.0f / 6.0f, 0.5f), 1.0f) : 1.0f - fexp2(x);
+  return __builtin_amdgcn_fmed3f(sc.A * om, 0.0f, sc.top);
+}
+
 __device__ __forceinline__ float3 lerp3(float4 a, float4 b, float s) {
   return make_float3(fmaf(s, b.x - a.x, a.x), fmaf(s, b.y - a.y, a.y), fmaf(s, b.z - a.z, a.z));
 }
@@ -753,25 +773,34 @@ constexpr int kStepGroup = 4;
 struct SampleGroup {
   float4 T[kStepGroup];
   float2 ab[kStepGroup];
+  double2 th[kStepGroup];  // t_i, h_i (the exact sun coordinate only)
 };
 
-__device__ __forceinline__ void load_sample_group(const float4* __restrict__ gT, const float4* __restrict__ gRS, int i,
-                                                  int n, SampleGroup& g) {
+template <bool kExact>
+__device__ __forceinline__ void load_sample_group(const float4* __restrict__ gT, const float4* __restrict__ gRS,
+                                                  const double2* __restrict__ gTH, int i, int n, SampleGroup& g) {
 #pragma unroll
   for (int q = 0; q < kStepGroup; ++q) {
     const int j = min(i + q, n - 1);  // in bounds
     g.T[q] = gT[j];
     const float4 r = gRS[j];
     g.ab[q] = make_float2(r.z, r.w);
+    if (kExact) g.th[q] = gTH[j];
   }
 }
 
+// The entry's sun-elevation coordinate terms: FP32 (xl, vl) and, for the exact form, FP64 ones and R.
+struct SunTerms {
+  float xl, vl;
+  double xl64, vl64, R;
+};
+
 // ray_scatter_samples over samples b .. min(b + kStepGroup, qn) - 1 of the chunk (b a multiple of
 // kStepGroup), their terms in g: the same operations in the same order.
-template <bool kColumn>
+template <bool kColumn, bool kExact>
 __device__ __forceinline__ float3 ray_scatter_sample_group(const SunCoord& sc, const float4* s_slab,
                                                            const SampleGroup& g, int b, int qn, float row_bytes,
-                                                           float col_bytes, int NLA, float vl, float xl, Lerp La,
+                                                           float col_bytes, int NLA, const SunTerms& st, Lerp La,
                                                            float3 acc) {
   const unsigned rb = (unsigned)row_bytes;
   const float slab_bytes = (float)(NLA * (int)sizeof(float4));
@@ -780,7 +809,8 @@ __device__ __forceinline__ float3 ray_scatter_sample_group(const SunCoord& sc, c
   for (int q = 0; q < kStepGroup; ++q, q_bytes += slab_bytes) {
     if (b + q < qn) {
       const float4 w = g.T[q];
-      const float c = sun_coord(sc, g.ab[q], xl, vl);
+      const float c = kExact ? sun_coord_exact(sc, st.R, g.th[q], st.xl64, st.vl64)
+                             : sun_coord(sc, g.ab[q], st.xl, st.vl);
       const float row = lower_cell(c, sc.top - 1.0f);
       const float s = c - row;
       const unsigned off = (unsigned)fmaf(row, row_bytes, q_bytes);
@@ -801,11 +831,12 @@ __device__ __forceinline__ float3 ray_scatter_sample_group(const SunCoord& sc, c
   return acc;
 }
 
-template <bool kColumn>
+template <bool kColumn, bool kExact>
 __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const float4* s_slab,
                                                       const float4* __restrict__ gT, const float4* __restrict__ gRS,
-                                                      int i0, int qn, float row_bytes, float col_bytes, int NLA,
-                                                      float vl, float xl, Lerp La, float3 acc) {
+                                                      const double2* __restrict__ gTH, int i0, int qn, float row_bytes,
+                                                      float col_bytes, int NLA, const SunTerms& st, Lerp La,
+                                                      float3 acc) {
   const unsigned rb = (unsigned)row_bytes;
   const float slab_bytes = (float)(NLA * (int)sizeof(float4));
   float q_bytes = col_bytes;  // byte offset of the entry's column in slab q
@@ -813,7 +844,8 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
     const int i = i0 + q;
     const float4 w = gT[i];  // workgroup-uniform: scalar loads
     const float4 g = gRS[i];
-    const float c = sun_coord(sc, make_float2(g.z, g.w), xl, vl);
+    const float c = kExact ? sun_coord_exact(sc, st.R, gTH[i], st.xl64, st.vl64)
+                           : sun_coord(sc, make_float2(g.z, g.w), st.xl, st.vl);
     const float row = lower_cell(c, sc.top - 1.0f);
     const float s = c - row;
     const unsigned off = (unsigned)fmaf(row, row_bytes, q_bytes);
@@ -981,7 +1013,7 @@ constexpr int kRSHalf = kRSBlock / 2;
 
 // Eight waves per SIMD (four workgroups per CU, as the LDS allows): at most 64 VGPRs, the re-sampling
 // of the accumulations spilling a few registers outside the loops.
-template <int EPT, bool kB>
+template <int EPT, bool kB, bool kExact>
 __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) ray_scatter_kernel(
     const DevParams P0, const DevParams* __restrict__ PB, const float4* __restrict__ dJ,
     const double2* __restrict__ geo_th, const float4* __restrict__ geo_T, const float4* __restrict__ geo_rs,
@@ -1068,57 +1100,66 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
   const SunCoord sc = {(float)((NL - 1) * (double)P.sun_inv_den), (float)(-0.6 * 1.4426950408889634), (float)(NL - 1)};
   const float row_bytes = (float)(NA * (int)sizeof(float4));
   // per-entry constants: heading coordinate and the terms of dot(p_i, l)
-  float vl[EPT], xl[EPT];
+  SunTerms st[EPT];
   Lerp La[EPT];
   float3 acc[EPT];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     int k = t + q * kRSHalf;
     acc[q] = make_float3(0.f, 0.f, 0.f);
-    vl[q] = 0.f;
-    xl[q] = 0.f;
+    st[q].vl = 0.f;
+    st[q].xl = 0.f;
+    st[q].vl64 = 0.0;
+    st[q].xl64 = 0.0;
+    st[q].R = P.g.R;
     La[q] = make_lerp(0.f, NA);
     if (k < NLA) {
       // the per-build sun terms (v . l, x . l, the heading coordinate, rounded once from FP64)
       const float4 m = P.sun_rs[(long long)rayid * NLA + k];
-      vl[q] = m.x;
-      xl[q] = m.y;
+      st[q].vl = m.x;
+      st[q].xl = m.y;
       La[q] = make_lerp(m.z, NA);
+      if (kExact) {  // the same dot products in FP64
+        const double2 d = P.sun_ss[(long long)rayid * NLA + k];
+        st[q].vl64 = d.x;
+        st[q].xl64 = d.y;
+      }
     }
   }
   SampleGroup sgp;
-  if constexpr (EPT == 1) load_sample_group(gT, gRS, 0, n, sgp);
+  if constexpr (EPT == 1) load_sample_group<kExact>(gT, gRS, geo_th + (long long)rayid * n, 0, n, sgp);
   __syncthreads();  // chunk 0 folded
   for (int c = 0; c < chunks; ++c) {
     const int i0 = c * Q, qn = min(Q, n - i0);
     const float4* slab = s_slab + (c & 1) * Q * NLA;
     if constexpr (EPT == 1) {
       for (int b = 0; b < qn; b += kStepGroup) {
-        if (b > 0) load_sample_group(gT, gRS, i0 + b, n, sgp);
+        if (b > 0) load_sample_group<kExact>(gT, gRS, geo_th + (long long)rayid * n, i0 + b, n, sgp);
         if (t < NLA) {
           const float col_bytes = (float)(La[0].i0 * (int)sizeof(float4));
           // a heading coordinate with zero weight (every entry but the clamped ones) reads one column;
           // the choice is made per wavefront so that a wavefront never runs both loops
           if (__all(La[0].s == 0.0f))
-            acc[0] = ray_scatter_sample_group<true>(sc, slab, sgp, b, qn, row_bytes, col_bytes, NLA, vl[0], xl[0],
-                                                    La[0], acc[0]);
+            acc[0] = ray_scatter_sample_group<true, kExact>(sc, slab, sgp, b, qn, row_bytes, col_bytes, NLA, st[0],
+                                                            La[0], acc[0]);
           else
-            acc[0] = ray_scatter_sample_group<false>(sc, slab, sgp, b, qn, row_bytes, col_bytes, NLA, vl[0], xl[0],
-                                                     La[0], acc[0]);
+            acc[0] = ray_scatter_sample_group<false, kExact>(sc, slab, sgp, b, qn, row_bytes, col_bytes, NLA, st[0],
+                                                             La[0], acc[0]);
         }
       }
-      if (c + 1 < chunks) load_sample_group(gT, gRS, i0 + qn, n, sgp);  // in flight across the barrier
+      if (c + 1 < chunks)  // in flight across the barrier
+        load_sample_group<kExact>(gT, gRS, geo_th + (long long)rayid * n, i0 + qn, n, sgp);
     } else {  // several entries per lane: the terms per sample (a group would spill)
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
         if (t + e * kRSHalf < NLA) {
           const float col_bytes = (float)(La[e].i0 * (int)sizeof(float4));
           if (__all(La[e].s == 0.0f))
-            acc[e] = ray_scatter_samples<true>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
-                                               La[e], acc[e]);
+            acc[e] = ray_scatter_samples<true, kExact>(sc, slab, gT, gRS, geo_th + (long long)rayid * n, i0, qn,
+                                                       row_bytes, col_bytes, NLA, st[e], La[e], acc[e]);
           else
-            acc[e] = ray_scatter_samples<false>(sc, slab, gT, gRS, i0, qn, row_bytes, col_bytes, NLA, vl[e], xl[e],
-                                                La[e], acc[e]);
+            acc[e] = ray_scatter_samples<false, kExact>(sc, slab, gT, gRS, geo_th + (long long)rayid * n, i0, qn,
+                                                        row_bytes, col_bytes, NLA, st[e], La[e], acc[e]);
         }
       }
     }
@@ -2073,6 +2114,37 @@ int ray_scatter_chunk(const DevParams& P) {
   return q < 1 ? 1 : (q > qmax ? std::max(qmax, 1) : q);
 }
 
+// launch_ray_scatter with the sun-elevation coordinate's form chosen (kExact: sun_coord_exact)
+template <bool kExact>
+hipError_t launch_ray_scatter_as(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
+                                 hipStream_t stream, const RsAccum& acc, unsigned rays, int Q, size_t shmem,
+                                 const RayGeoView& V, const DevParams* PB) {
+  const int nla = P.nS[2] * P.nS[3];
+  if (P.batch > 0) {
+    const dim3 grid(rays, 1, (unsigned)P.batch);
+    if (nla <= kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<1, true, kExact>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ,
+                         V.th, V.T, V.rs, V.fold, out, acc, Q, wc);
+    else if (nla <= 2 * kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<2, true, kExact>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ,
+                         V.th, V.T, V.rs, V.fold, out, acc, Q, wc);
+    else
+      hipLaunchKernelGGL((ray_scatter_kernel<4, true, kExact>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ,
+                         V.th, V.T, V.rs, V.fold, out, acc, Q, wc);
+  } else {
+    if (nla <= kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<1, false, kExact>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ,
+                         V.th, V.T, V.rs, V.fold, out, acc, Q, wc);
+    else if (nla <= 2 * kRSHalf)
+      hipLaunchKernelGGL((ray_scatter_kernel<2, false, kExact>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ,
+                         V.th, V.T, V.rs, V.fold, out, acc, Q, wc);
+    else
+      hipLaunchKernelGGL((ray_scatter_kernel<4, false, kExact>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ,
+                         V.th, V.T, V.rs, V.fold, out, acc, Q, wc);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out, WorkCounters* wc,
                               hipStream_t stream, const RsAccum& acc) {
   if ((acc.S_in == nullptr) != (acc.S_out == nullptr) || (acc.E_out && !acc.dE)) return hipErrorInvalidValue;
@@ -2084,29 +2156,11 @@ hipError_t launch_ray_scatter(const DevParams& P, const float4* dJ, float4* out,
   const RayGeoView V = ray_geo_view(P);
   const DevParams* PB = P.batch_params;
   if (nla > 4 * kRSHalf) return hipErrorInvalidValue;
-  if (P.batch > 0) {
-    const dim3 grid(rays, 1, (unsigned)P.batch);
-    if (nla <= kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<1, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
-                         V.rs, V.fold, out, acc, Q, wc);
-    else if (nla <= 2 * kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<2, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
-                         V.rs, V.fold, out, acc, Q, wc);
-    else
-      hipLaunchKernelGGL((ray_scatter_kernel<4, true>), grid, dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
-                         V.rs, V.fold, out, acc, Q, wc);
-  } else {
-    if (nla <= kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<1, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
-                         V.rs, V.fold, out, acc, Q, wc);
-    else if (nla <= 2 * kRSHalf)
-      hipLaunchKernelGGL((ray_scatter_kernel<2, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
-                         V.rs, V.fold, out, acc, Q, wc);
-    else
-      hipLaunchKernelGGL((ray_scatter_kernel<4, false>), dim3(rays), dim3(kRSBlock), shmem, stream, P, PB, dJ, V.th, V.T,
-                         V.rs, V.fold, out, acc, Q, wc);
+  if (P.nS[2] < kExactSunRows) {
+    if (!P.sun_ss) return hipErrorInvalidValue;
+    return launch_ray_scatter_as<true>(P, dJ, out, wc, stream, acc, rays, Q, shmem, V, PB);
   }
-  return hipGetLastError();
+  return launch_ray_scatter_as<false>(P, dJ, out, wc, stream, acc, rays, Q, shmem, V, PB);
 }
 
 hipError_t launch_surface_geometry(const DevParams& P, const Dir* dirs, int ndirs, hipStream_t stream) {

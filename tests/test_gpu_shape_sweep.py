@@ -5,8 +5,10 @@ oracle's at the parity tolerances.
 The reference's tests fix a handful of shapes; this sweep takes sizes from the smallest the interface
 accepts (2 per dimension, 1 view sample) upward, odd and even, so that chunking, the per-lane entry
 split, the parity split of the fold and the horizon rows meet grids that no hand-picked case covers.
-It found the grazing ray's NaN at odd sample counts and the fused multiply-adds in the FP64 index round
-trips (coarse transmittance grids, up to 7e-3), both fixed; one case stays a documented FP32 limit.
+It found the grazing ray's NaN at odd sample counts, the fused multiply-adds in the FP64 index round
+trips (coarse transmittance grids, up to 7e-3) and the FP32 sun-elevation coordinate of ray scattering on
+coarse sun-elevation grids (up to 3.6e-3), all fixed.  Sun-elevation grids of fewer than 16 rows take the
+exact form of that coordinate; the second sweep keeps the FP32 form covered on small ragged grids.
 """
 import numpy as np
 import pytest
@@ -16,7 +18,7 @@ from tests.parity import TOL_SCATTER, TOL_TRANSMITTANCE, assert_close, oracle_pa
 pytestmark = pytest.mark.gpu
 
 
-def sweep_cases(count=40, seed=7):
+def sweep_cases(count=40, seed=7, sun_rows=(2, 13)):
     rng = np.random.default_rng(seed)
     cases = []
     for _ in range(count):
@@ -25,7 +27,7 @@ def sweep_cases(count=40, seed=7):
             height=float(rng.choice([35000.0, 60000.0, 100000.0])),
             transmittance_shape=(int(rng.integers(2, 17)), int(rng.integers(2, 33))),
             surface_radiance_shape=(int(rng.integers(2, 9)), int(rng.integers(2, 17))),
-            scatter_shape=(int(rng.integers(2, 10)), int(rng.integers(2, 15)), int(rng.integers(2, 13)),
+            scatter_shape=(int(rng.integers(2, 10)), int(rng.integers(2, 15)), int(rng.integers(*sun_rows)),
                            int(rng.integers(2, 13))),
             # the surface-radiance half sphere takes ray_steps >> 2 rings: at least 4 samples with iterations
             ray_steps=int(rng.integers(4 if iterations else 1, 41)),
@@ -57,26 +59,21 @@ def case_id(c):
     return "x".join(map(str, c["scatter_shape"])) + f"-n{c['ray_steps']}-s{c['sphere_steps']}-it{c['iterations']}"
 
 
-# The ray-scatter lookup evaluates the sun-elevation coordinate in FP32 as A - A exp2(arg).  Where a view
-# ray is vertical and the sun sits at the bottom of the index range (sin = -0.2, where the coordinate is
-# clamped at 0 and arg cancels to 0), the coordinate carries ~1e-6 index units of rounding, a weight that
-# the FP64 oracle does not have on the next sun-elevation row; on a 12-row grid that row is orders of
-# magnitude brighter, and an entry at 2e-6 of the table's maximum (just above the comparison floor) is off
-# by 1.6e-3.  Production grids (32 rows) stay well inside the bound.
-FP32_SUN_COORDINATE = {"2x4x12x9-n36-s4-it1"}
+# Shapes whose ray scattering exceeded the bound with the sun-elevation coordinate in FP32 (a 12-row and two
+# 2-row sun-elevation grids, 1.6e-3, 3.6e-3 and 2.7e-3): at the bottom of the index range (sin = -0.2, where
+# the coordinate is clamped at 0) its exponent cancels to 0 with ~3e-7 index units of rounding, a weight on
+# the next row, which is orders of magnitude brighter on such grids.  They now take the exact form.
+COARSE_SUN_GRIDS = [
+    dict(height=60000.0, transmittance_shape=(7, 9), surface_radiance_shape=(8, 2), scatter_shape=(2, 4, 12, 9),
+         ray_steps=36, sphere_steps=4, iterations=1),
+    dict(height=35000.0, transmittance_shape=(3, 12), surface_radiance_shape=(7, 8), scatter_shape=(9, 9, 2, 5),
+         ray_steps=20, sphere_steps=8, iterations=1),
+    dict(height=60000.0, transmittance_shape=(4, 4), surface_radiance_shape=(5, 7), scatter_shape=(8, 11, 2, 3),
+         ray_steps=34, sphere_steps=10, iterations=1),
+]
 
 
-def marked(cases):
-    out = []
-    for c in cases:
-        marks = [pytest.mark.xfail(strict=True, reason="FP32 sun-elevation coordinate at the index clamp, "
-                                   "12-row grid")] if case_id(c) in FP32_SUN_COORDINATE else []
-        out.append(pytest.param(c, id=case_id(c), marks=marks))
-    return out
-
-
-@pytest.mark.parametrize("case", marked(sweep_cases()))
-def test_sweep_against_oracle(atm, gpu, oracle, case):
+def check_case(atm, gpu, oracle, case):
     c = dict(case)
     P = atm.make_params(planet={"height": c.pop("height"), "brightness": (0.3,) * 3}, **c)
     T, E, S, M = oracle.generate_atmosphere_luts(oracle_params(oracle, P))
@@ -88,6 +85,22 @@ def test_sweep_against_oracle(atm, gpu, oracle, case):
         assert not got["surface-radiance"].any()
     assert_close("ray-scatter", got["ray-scatter"], oracle.convert_4d_to_2d(S), TOL_SCATTER)
     assert_close("mie-strength", got["mie-strength"], oracle.convert_4d_to_2d(M), TOL_SCATTER)
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=case_id(c)) for c in sweep_cases()])
+def test_sweep_against_oracle(atm, gpu, oracle, case):
+    check_case(atm, gpu, oracle, case)
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=case_id(c)) for c in COARSE_SUN_GRIDS])
+def test_coarse_sun_grids_against_oracle(atm, gpu, oracle, case):
+    check_case(atm, gpu, oracle, case)
+
+
+@pytest.mark.parametrize("case", [pytest.param(c, id=case_id(c)) for c in sweep_cases(12, 5, sun_rows=(16, 21))])
+def test_sweep_fp32_sun_coordinate_against_oracle(atm, gpu, oracle, case):
+    """Sun-elevation grids of 16 rows and more keep the FP32 form of the coordinate."""
+    check_case(atm, gpu, oracle, case)
 
 
 TABLES = ["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]
