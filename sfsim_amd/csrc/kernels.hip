@@ -737,23 +737,24 @@ __device__ __forceinline__ float sun_coord(const SunCoord& sc, float2 ab, float 
 }
 
 // Sun-elevation grids with fewer rows than this take the exact form below.  Their rows lie far apart in
-// sun elevation, so the row above a sun just below the index range's lower end (sin = -0.2, where the
-// coordinate is clamped at 0) can be brighter by orders of magnitude, and sun_coord's FP32 exponent,
-// which cancels to 0 there (rounding ~3e-7 index units), would weight that row by as much.
+// sun elevation, so neighbouring rows can differ by orders of magnitude (a sun below the horizon against
+// one above it), and the ~1e-7 index units of rounding in sun_coord's FP32 form weight such a row by as
+// much: where the true coordinate is integral (a vertical ray keeps its sun elevation) or clamped (the
+// exponent cancels to 0 at sin = -0.2), the FP64 oracle gives that row no weight at all.
 constexpr int kExactSunRows = 16;
 
-// sun_coord with the exponent x = log2(e) (-3 dot(p_i, l) / r_i - 0.6) formed in FP64 from the sample's
-// t_i, h_i and the entry's dot(x, l), dot(v, l) (1 / r_i: the FP32 estimate and one FP64 Newton step,
-// relative error ~1e-14), and 1 - 2^x from its series where |x| < 1/32 (no cancellation near the clamp).
-__device__ __forceinline__ float sun_coord_exact(const SunCoord& sc, double R, double2 th, double xl, double vl) {
+// sun_coord in FP64: z = -3 dot(p_i, l) / r_i - 0.6 from the sample's t_i, h_i and the entry's dot(x, l),
+// dot(v, l) (1 / r_i: the FP32 estimate and one FP64 Newton step, relative error ~1e-14), the coordinate
+// (NL - 1) (1 - e^z) / (1 - e^-3.6) with 1 - e^z = -expm1(z) (no cancellation near the clamp), clamped and
+// rounded to FP32 once.  A64: (NL - 1) / (1 - e^-3.6).
+__device__ __forceinline__ float sun_coord_exact(const SunCoord& sc, double A64, double R, double2 th, double xl,
+                                                 double vl) {
   const double r = R + th.y;
   double inv = (double)__builtin_amdgcn_rcpf((float)r);
   inv = fma(inv, fma(-r, inv, 1.0), inv);
-  const double a = -3.0 * 1.4426950408889634 * inv;
-  const float x = (float)fma(a * th.x, vl, fma(a, xl, -0.6 * 1.4426950408889634));
-  const float y = x * 0.6931471805599453f;
-  const float om = fabsf(x) < 0.03125f ? -y * fmaf(y, fmaf(y, 1.0f / 6.0f, 0.5f), 1.0f) : 1.0f - fexp2(x);
-  return __builtin_amdgcn_fmed3f(sc.A * om, 0.0f, sc.top);
+  const double a = -3.0 * inv;
+  const double z = fma(a * th.x, vl, fma(a, xl, -0.6));
+  return (float)fmin(fmax(-expm1(z) * A64, 0.0), (double)sc.top);
 }
 
 __device__ __forceinline__ float3 lerp3(float4 a, float4 b, float s) {
@@ -789,10 +790,10 @@ __device__ __forceinline__ void load_sample_group(const float4* __restrict__ gT,
   }
 }
 
-// The entry's sun-elevation coordinate terms: FP32 (xl, vl) and, for the exact form, FP64 ones and R.
+// The entry's sun-elevation coordinate terms: FP32 (xl, vl) and, for the exact form, FP64 ones, R and A64.
 struct SunTerms {
   float xl, vl;
-  double xl64, vl64, R;
+  double xl64, vl64, R, A64;
 };
 
 // ray_scatter_samples over samples b .. min(b + kStepGroup, qn) - 1 of the chunk (b a multiple of
@@ -809,7 +810,7 @@ __device__ __forceinline__ float3 ray_scatter_sample_group(const SunCoord& sc, c
   for (int q = 0; q < kStepGroup; ++q, q_bytes += slab_bytes) {
     if (b + q < qn) {
       const float4 w = g.T[q];
-      const float c = kExact ? sun_coord_exact(sc, st.R, g.th[q], st.xl64, st.vl64)
+      const float c = kExact ? sun_coord_exact(sc, st.A64, st.R, g.th[q], st.xl64, st.vl64)
                              : sun_coord(sc, g.ab[q], st.xl, st.vl);
       const float row = lower_cell(c, sc.top - 1.0f);
       const float s = c - row;
@@ -844,7 +845,7 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
     const int i = i0 + q;
     const float4 w = gT[i];  // workgroup-uniform: scalar loads
     const float4 g = gRS[i];
-    const float c = kExact ? sun_coord_exact(sc, st.R, gTH[i], st.xl64, st.vl64)
+    const float c = kExact ? sun_coord_exact(sc, st.A64, st.R, gTH[i], st.xl64, st.vl64)
                            : sun_coord(sc, make_float2(g.z, g.w), st.xl, st.vl);
     const float row = lower_cell(c, sc.top - 1.0f);
     const float s = c - row;
@@ -1112,6 +1113,7 @@ __global__ void __launch_bounds__(kRSBlock) __attribute__((amdgpu_waves_per_eu(8
     st[q].vl64 = 0.0;
     st[q].xl64 = 0.0;
     st[q].R = P.g.R;
+    st[q].A64 = kExact ? (NL - 1) / (1.0 - exp(-3.6)) : 0.0;
     La[q] = make_lerp(0.f, NA);
     if (k < NLA) {
       // the per-build sun terms (v . l, x . l, the heading coordinate, rounded once from FP64)
