@@ -745,16 +745,20 @@ constexpr int kExactSunRows = 16;
 
 // sun_coord in FP64: z = -3 dot(p_i, l) / r_i - 0.6 from the sample's t_i, h_i and the entry's dot(x, l),
 // dot(v, l) (1 / r_i: the FP32 estimate and one FP64 Newton step, relative error ~1e-14), the coordinate
-// (NL - 1) (1 - e^z) / (1 - e^-3.6) with 1 - e^z = -expm1(z) (no cancellation near the clamp), clamped and
-// rounded to FP32 once.  A64: (NL - 1) / (1 - e^-3.6).
-__device__ __forceinline__ float sun_coord_exact(const SunCoord& sc, double A64, double R, double2 th, double xl,
-                                                 double vl) {
+// (NL - 1) (1 - e^z) / (1 - e^-3.6) with 1 - e^z = -expm1(z) (no cancellation near the clamp), clamped, and
+// its lower row and weight, the weight formed in FP64 before rounding (a coordinate just above a row, say
+// 1 + 5e-6, would lose 1% of its weight to the FP32 spacing at 1).  A64: (NL - 1) / (1 - e^-3.6).
+__device__ __forceinline__ void sun_coord_exact(const SunCoord& sc, double A64, double R, double2 th, double xl,
+                                                double vl, float& row, float& s) {
   const double r = R + th.y;
   double inv = (double)__builtin_amdgcn_rcpf((float)r);
   inv = fma(inv, fma(-r, inv, 1.0), inv);
   const double a = -3.0 * inv;
   const double z = fma(a * th.x, vl, fma(a, xl, -0.6));
-  return (float)fmin(fmax(-expm1(z) * A64, 0.0), (double)sc.top);
+  const double c = fmin(fmax(-expm1(z) * A64, 0.0), (double)sc.top);
+  const double r0 = fmin(floor(c), (double)sc.top - 1.0);
+  row = (float)r0;
+  s = (float)(c - r0);
 }
 
 __device__ __forceinline__ float3 lerp3(float4 a, float4 b, float s) {
@@ -810,10 +814,14 @@ __device__ __forceinline__ float3 ray_scatter_sample_group(const SunCoord& sc, c
   for (int q = 0; q < kStepGroup; ++q, q_bytes += slab_bytes) {
     if (b + q < qn) {
       const float4 w = g.T[q];
-      const float c = kExact ? sun_coord_exact(sc, st.A64, st.R, g.th[q], st.xl64, st.vl64)
-                             : sun_coord(sc, g.ab[q], st.xl, st.vl);
-      const float row = lower_cell(c, sc.top - 1.0f);
-      const float s = c - row;
+      float row, s;
+      if (kExact) {
+        sun_coord_exact(sc, st.A64, st.R, g.th[q], st.xl64, st.vl64, row, s);
+      } else {
+        const float c = sun_coord(sc, g.ab[q], st.xl, st.vl);
+        row = lower_cell(c, sc.top - 1.0f);
+        s = c - row;
+      }
       const unsigned off = (unsigned)fmaf(row, row_bytes, q_bytes);
       float3 v;
       if (kColumn) {
@@ -845,10 +853,14 @@ __device__ __forceinline__ float3 ray_scatter_samples(const SunCoord& sc, const 
     const int i = i0 + q;
     const float4 w = gT[i];  // workgroup-uniform: scalar loads
     const float4 g = gRS[i];
-    const float c = kExact ? sun_coord_exact(sc, st.A64, st.R, gTH[i], st.xl64, st.vl64)
-                           : sun_coord(sc, make_float2(g.z, g.w), st.xl, st.vl);
-    const float row = lower_cell(c, sc.top - 1.0f);
-    const float s = c - row;
+    float row, s;
+    if (kExact) {
+      sun_coord_exact(sc, st.A64, st.R, gTH[i], st.xl64, st.vl64, row, s);
+    } else {
+      const float c = sun_coord(sc, make_float2(g.z, g.w), st.xl, st.vl);
+      row = lower_cell(c, sc.top - 1.0f);
+      s = c - row;
+    }
     const unsigned off = (unsigned)fmaf(row, row_bytes, q_bytes);
     float3 v;
     if (kColumn) {
