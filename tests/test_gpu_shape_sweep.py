@@ -116,6 +116,17 @@ def test_sweep_fp32_sun_coordinate_against_oracle(atm, gpu, oracle, case):
     check_case(atm, gpu, oracle, case)
 
 
+# Wider sweeps with other seeds (a second of GPU time each): 2-12 sun-elevation rows, and 13-32 rows across
+# the switch between the exact and the FP32 forms of the sun-elevation coordinate.
+WIDE = [pytest.param(c, id="w-" + case_id(c)) for c in sweep_cases(200, 31337)] + \
+       [pytest.param(c, id="w-" + case_id(c)) for c in sweep_cases(100, 555, sun_rows=(13, 33))]
+
+
+@pytest.mark.parametrize("case", WIDE)
+def test_wide_sweep_against_oracle(atm, gpu, oracle, case):
+    check_case(atm, gpu, oracle, case)
+
+
 TABLES = ["transmittance", "surface-radiance", "ray-scatter", "mie-strength"]
 
 
